@@ -1,0 +1,270 @@
+"""The oracle pinned against golden vectors, the reference's own symmetrize, closed forms and
+finite differences (CPU only)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from ncvis_amd import synthetic
+
+
+@pytest.fixture(scope="module")
+def rng_golden(golden_dir):
+    with open(os.path.join(golden_dir, "rng_golden.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("seed", [42, 43, 0])
+def test_pcg64_raw_matches_pcg_cpp(rng_golden, seed):
+    want = np.array([int(v) for v in rng_golden[f"pcg64_seed{seed}"]], np.uint64)
+    assert np.array_equal(oracle.pcg64_stream(seed, len(want)), want)
+
+
+@pytest.mark.parametrize("key", ["uniform_index_seed42_n5620", "uniform_index_seed42_n1000000",
+                                 "uniform_index_seed7_n100", "uniform_index_seed42_n2p63p1"])
+def test_uniform_index_matches_libstdcxx(rng_golden, key):
+    g = rng_golden[key]
+    seed = int(key.split("seed")[1].split("_")[0])
+    want = np.array([int(v) for v in g["draws"]], np.uint64)
+    got = oracle.uniform_index_stream(seed, int(g["range"]), len(want))
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("seed", [42, 45])
+def test_uniform_float_matches_libstdcxx(rng_golden, seed):
+    want = np.array(rng_golden[f"uniform_float_bits_seed{seed}"], np.uint32)
+    got = oracle.uniform_float_stream(seed, len(want)).view(np.uint32)
+    assert np.array_equal(got, want)
+
+
+def test_static_chunk_partition():
+    # libgomp static schedule: contiguous, covering, sizes differ by at most one, bigger first
+    for n in [0, 1, 7, 100, 101]:
+        for T in [1, 2, 3, 8]:
+            spans = [oracle.static_chunk(n, T, t) for t in range(T)]
+            assert spans[0][0] == 0 and spans[-1][1] == n
+            for (a, b), (c, _) in zip(spans, spans[1:]):
+                assert b == c
+            sizes = [b - a for a, b in spans]
+            assert max(sizes) - min(sizes) <= 1 and sizes == sorted(sizes, reverse=True)
+
+
+@pytest.mark.parametrize("dist", ["euclidean", "inner_product", "cosine", "correlation"])
+def test_preprocess_closed_form(dist):
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(50, 17)).astype(np.float32)
+    X[3] = 0.0
+    got = oracle.preprocess(X, dist)
+    X64 = X.astype(np.float64)
+    if dist == "correlation":
+        X64 = X64 - X64.mean(1, keepdims=True)
+    if dist in ("cosine", "correlation"):
+        n = np.linalg.norm(X64, axis=1, keepdims=True)
+        X64 = np.where(n > 0, X64 / np.where(n > 0, n, 1), X64)
+    np.testing.assert_allclose(got, X64, rtol=2e-6, atol=2e-6)
+
+
+@pytest.mark.parametrize("dist", ["euclidean", "cosine"])
+def test_knn_bruteforce_vs_fp64(dist):
+    if dist == "euclidean":
+        X, _ = synthetic.digits_like(N=600, D=64, seed=2)  # integers: exact in float32
+    else:
+        X, _ = synthetic.normalized_mixture(N=600, D=32, n_clusters=5, seed=2)
+    Xp = oracle.preprocess(X, dist)
+    i32, d32 = oracle.knn_bruteforce(Xp, 15, dist)
+    i64, d64 = oracle.knn_fp64(Xp, 15, dist)
+    if dist == "euclidean":
+        assert np.array_equal(i32, i64)
+        assert np.array_equal(d32, d64.astype(np.float32))
+    else:
+        assert (i32 == i64).mean() > 0.995
+        np.testing.assert_allclose(d32, d64, atol=1e-5)
+    # reference row order: farthest first (descending distance)
+    assert np.all(np.diff(d32, axis=1) <= 0)
+
+
+def _sym_fixture(golden_dir, name):
+    z = np.load(os.path.join(golden_dir, name))
+    return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("name", ["sym_digits1500.npz", "sym_ties_hubs.npz", "sym_unsorted.npz",
+                                  "sym_tiny.npz"])
+def test_symmetrize_matches_reference_fixture(golden_dir, name):
+    f = _sym_fixture(golden_dir, name)
+    row_ptr, col, dist = oracle.symmetrize(f["inds"].astype(np.int64), f["dists"])
+    assert np.array_equal(row_ptr, f["row_ptr"])
+    assert np.array_equal(col, f["col"])
+    assert np.array_equal(dist.view(np.uint32), f["dist"].view(np.uint32))
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (no /root/reference)")
+@pytest.mark.parametrize("seed", range(6))
+def test_symmetrize_matches_reference_random(seed):
+    rng = np.random.default_rng(100 + seed)
+    N, k = int(rng.integers(20, 400)), int(rng.integers(1, 20))
+    k = min(k, N - 1)
+    levels = int(rng.choice([2, 5, 1000]))
+    inds = np.stack([rng.choice(np.delete(np.arange(N), i), k, replace=False) for i in range(N)])
+    if seed % 2:  # hubs
+        hubs = rng.choice(N, 3, replace=False)
+        mask = rng.random((N, k)) < 0.3
+        for i in range(N):
+            for j in np.nonzero(mask[i])[0]:
+                h = hubs[j % 3]
+                if h != i and h not in inds[i]:
+                    inds[i, j] = h
+    dists = (rng.integers(1, levels + 1, (N, k)) / levels).astype(np.float32)
+    a = oracle.symmetrize(inds, dists)
+    b = oracle.ref_symmetrize(inds, dists, n_threads=1)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_symmetrize_dedup_quirk():
+    # own list [2 (0.2), 1 (0.1)], nothing incoming to row 0: neighbour 2 is lost
+    # (in-place de-duplication compares entries it has already overwritten)
+    inds = np.array([[2, 1], [2, 2], [1, 1]], np.int64)
+    dists = np.array([[0.2, 0.1], [0.5, 0.5], [0.5, 0.5]], np.float32)
+    row_ptr, col, dist = oracle.symmetrize(inds, dists)
+    assert list(col[row_ptr[0]:row_ptr[1]]) == [1]
+
+
+def test_symmetrize_invariants(golden_dir):
+    f = _sym_fixture(golden_dir, "sym_digits1500.npz")
+    inds, dists = f["inds"].astype(np.int64), f["dists"]
+    row_ptr, col, dist = oracle.symmetrize(inds, dists)
+    N, k = inds.shape
+    incoming = [[] for _ in range(N)]
+    for i in range(N):
+        for j in inds[i]:
+            incoming[j].append(i)
+    for i in range(N):
+        row = col[row_ptr[i]:row_ptr[i + 1]]
+        dr = dist[row_ptr[i]:row_ptr[i + 1]]
+        assert np.all(np.diff(dr) >= 0)
+        assert set(row) <= set(inds[i]) | set(incoming[i])
+        assert len(row) <= k + len(incoming[i])
+
+
+@pytest.mark.parametrize("n", [1, 5, 16, 17, 40, 200])
+def test_intro_sort_sorts(n):
+    rng = np.random.default_rng(n)
+    key = rng.integers(0, 4, n).astype(np.float32)
+    idx = oracle.intro_sort_idx(key)
+    assert sorted(idx) == list(range(n))
+    assert np.all(np.diff(key[idx]) >= 0)
+
+
+def test_init_propagation_is_sparse_product():
+    # one init epoch = (alpha * A @ Y0), standardised per dimension (population std)
+    import scipy.sparse as sp
+    f_rng = np.random.default_rng(5)
+    N, d = 80, 3
+    X = f_rng.normal(size=(N, 5)).astype(np.float32)
+    inds, dists = oracle.knn_bruteforce(X, 6)
+    row_ptr, col, _ = oracle.symmetrize(inds, dists)
+    Y0 = oracle.init_embedding(N, d, row_ptr, col, 1.0 / 6, 0, 42, 1)
+    want0 = oracle.uniform_float_stream(42, N * d).reshape(N, d)
+    assert np.array_equal(Y0, want0)
+    Y1 = oracle.init_embedding(N, d, row_ptr, col, 1.0 / 6, 1, 42, 1)
+    A = sp.csr_matrix((np.ones(len(col)), col, row_ptr), shape=(N, N))
+    Z = (1.0 / 6) * (A @ want0.astype(np.float64))
+    Z = (Z - Z.mean(0)) / Z.std(0)
+    np.testing.assert_allclose(Y1, Z, atol=2e-5)
+    Y7 = oracle.init_embedding(N, d, row_ptr, col, 1.0 / 6, 7, 42, 1)
+    np.testing.assert_allclose(Y7.mean(0), 0, atol=1e-5)
+    np.testing.assert_allclose(Y7.std(0), 1, atol=1e-4)
+
+
+@pytest.mark.parametrize("T", [2, 3])
+def test_init_random_partition_matches_threads(T):
+    # with T threads, element i of N*d is draw (i - s0) of pcg64(seed + t), t = its chunk
+    N, d, seed = 37, 2, 9
+    row_ptr = np.zeros(N + 1, np.int64)
+    col = np.zeros(0, np.int64)
+    Y = oracle.init_embedding(N, d, row_ptr, col, 1.0, 0, seed, T).ravel()
+    for t in range(T):
+        s0, e0 = oracle.static_chunk(N * d, T, t)
+        assert np.array_equal(Y[s0:e0], oracle.uniform_float_stream(seed + t, e0 - s0))
+
+
+@pytest.mark.parametrize("positive", [True, False])
+@pytest.mark.parametrize("ab", [(0.7668736385558027, 1.2225638162909107), (1.0, 1.0),
+                                (1.5769434602697652, 0.8950608778515733)])
+def test_sgd_sample_is_nce_gradient_step(positive, ab):
+    """One reference sample update equals -step * dL/dY (and Q -= alpha_Q * dL/dQ) for the NCE
+    loss, checked by float64 central differences of oracle.nce_loss."""
+    a, b = ab
+    rng = np.random.default_rng(11)
+    N, d, n = 6, 2, 3
+    Y = rng.normal(size=(N, d)).astype(np.float32)
+    Q, step = -0.7, 1e-3
+    i, o = 1, 4
+    Y1, Q1 = oracle.sgd_sample(Y, i, o, positive, n, Q, step, a, b, 1.0)
+    # direct float64 single-sample loss
+    def sample_loss(Yv, Qv):
+        Yv = Yv.astype(np.float64)
+        d2 = ((Yv[i] - Yv[o]) ** 2).sum()
+        s = -np.log1p(a * d2 ** b) - Qv - np.log(n)
+        return np.logaddexp(0, -s) if positive else np.logaddexp(0, s)
+    eps = 1e-6
+    g = np.zeros((N, d))
+    for r in (i, o):
+        for c in range(d):
+            Yp = Y.astype(np.float64).copy(); Yp[r, c] += eps
+            Ym = Y.astype(np.float64).copy(); Ym[r, c] -= eps
+            g[r, c] = (sample_loss(Yp, Q) - sample_loss(Ym, Q)) / (2 * eps)
+    gQ = (sample_loss(Y, Q + eps) - sample_loss(Y, Q - eps)) / (2 * eps)
+    np.testing.assert_allclose((Y1.astype(np.float64) - Y) / step, -g, rtol=2e-3, atol=1e-4)
+    np.testing.assert_allclose(Q - Q1, gQ, rtol=1e-4, atol=1e-6)
+
+
+def test_nce_loss_gradient_fd():
+    rng = np.random.default_rng(12)
+    N, d, n, E = 12, 2, 3, 20
+    Y = rng.normal(size=(N, d)).astype(np.float32)
+    src = rng.integers(0, N, E); dst = (src + 1 + rng.integers(0, N - 1, E)) % N
+    neg = rng.integers(0, N, (E, n))
+    a, b, Q = 0.7668736385558027, 1.2225638162909107, -1.3
+    L, dY, dQ = oracle.nce_loss(Y, Q, src, dst, neg, a, b)
+    eps = 1e-3
+    for r, c in [(0, 0), (3, 1), (7, 0)]:
+        Yp = Y.copy(); Yp[r, c] += eps
+        Ym = Y.copy(); Ym[r, c] -= eps
+        fd = (oracle.nce_loss(Yp, Q, src, dst, neg, a, b, grad=False)
+              - oracle.nce_loss(Ym, Q, src, dst, neg, a, b, grad=False)) / (2 * eps)
+        assert abs(fd - dY[r, c]) < 2e-3 * max(1, abs(fd))
+    fdQ = (oracle.nce_loss(Y, Q + 1e-5, src, dst, neg, a, b, grad=False)
+           - oracle.nce_loss(Y, Q - 1e-5, src, dst, neg, a, b, grad=False)) / 2e-5
+    assert abs(fdQ - dQ) < 1e-4 * max(1, abs(fdQ))
+
+
+def test_optimize_step_schedule_closed_form():
+    # one edge each way, n_noise = 0 -> w = 1: pure attraction, per sample the gap shrinks by
+    # (1 - 2 step_e); step_e = alpha (1 - (e/E)^2) (ncvis.cpp:287)
+    Y = np.array([[0.0], [1.0]], np.float32)
+    row_ptr = np.array([0, 1, 2]); col = np.array([1, 0])
+    n_epochs, alpha = 4, 0.1
+    Y1, Q1 = oracle.optimize(Y, 0.0, row_ptr, col, np.zeros(n_epochs, np.uint64), 1.0, 1.0, alpha,
+                             1.0, 42, 1)
+    gap = 1.0
+    for e in range(n_epochs):
+        step = alpha * (1 - (e / n_epochs) ** 2)
+        gap *= (1 - 2 * step) ** 2
+    assert abs((Y1[1, 0] - Y1[0, 0]) - gap) < 1e-6
+    assert abs(Y1.mean() - 0.5) < 1e-6 and Q1 == 0.0
+
+
+def test_optimize_deterministic_single_thread():
+    X, _ = synthetic.digits_like(N=200, D=16, seed=4)
+    inds, dists = oracle.knn_bruteforce(X, 5)
+    row_ptr, col, _ = oracle.symmetrize(inds, dists)
+    Y0 = oracle.init_embedding(200, 2, row_ptr, col, 0.2, 4, 42, 1)
+    plan = np.array([1, 2, 3], np.uint64)
+    a = oracle.optimize(Y0, 0.0, row_ptr, col, plan, 0.77, 1.22, 1.0, 1.0, 42, 1)
+    b = oracle.optimize(Y0, 0.0, row_ptr, col, plan, 0.77, 1.22, 1.0, 1.0, 42, 1)
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+    assert np.all(np.isfinite(a[0]))
