@@ -1,0 +1,105 @@
+/*
+ * C ABI of the MI355X-native NCVis fit path (libncvis_amd.so, gfx950).
+ *
+ * Plain pointers and sizes only. Streams are passed as `void*` holding a hipStream_t (NULL =
+ * the library's own stream for handle calls, the default stream for stage calls). Every entry
+ * point returns 0 on success and a negative status on failure; the message is available from
+ * ncvis_last_error() (handle calls) or ncvis_get_last_error() (stage calls, per thread).
+ *
+ * Reference interfaces these replace (paths relative to the reference repository):
+ *   ncvis_create / ncvis_destroy  -> ncvis::NCVis::NCVis / ~NCVis (src/ncvis.hpp:54-55,
+ *                                    src/ncvis.cpp:12-30)
+ *   ncvis_fit_transform           -> ncvis::NCVis::fit_transform (src/ncvis.hpp:66,
+ *                                    src/ncvis.cpp:354-459), host buffers
+ *   ncvis_knn_device              -> buildKNN + findKNN (src/ncvis.cpp:72-153)
+ *   ncvis_symmetrize_device       -> KNNTable::symmetrize + build_edges (src/knntable.cpp:24-99,
+ *                                    src/ncvis.cpp:155-173)
+ *   ncvis_init_embedding_device   -> init_embedding (src/ncvis.cpp:184-273)
+ *   ncvis_optimize_device         -> optimize (src/ncvis.cpp:275-352)
+ *   ncvis_nce_loss_device         -> no reference function: the NCE objective whose gradient
+ *                                    src/ncvis.cpp:304-338 applies in place
+ */
+#ifndef NCVIS_AMD_H
+#define NCVIS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Distance codes: the ordinals of ncvis::Distance (src/ncvis.hpp:27-32). */
+enum ncvis_distance { NCVIS_SQUARED_L2 = 0, NCVIS_INNER_PRODUCT = 1, NCVIS_COSINE = 2, NCVIS_CORRELATION = 3 };
+
+/* Candidate-GEMM precision of the kNN search; final distances are always exact FP32. */
+enum ncvis_precision { NCVIS_PREC_FP32 = 0, NCVIS_PREC_BF16 = 1, NCVIS_PREC_FP16 = 2 };
+
+/* Status codes */
+enum ncvis_status { NCVIS_OK = 0, NCVIS_EINVAL = -1, NCVIS_EHIP = -2, NCVIS_EINTERNAL = -3 };
+
+typedef struct ncvis_handle ncvis_handle;
+
+/* ncvis::NCVis::NCVis (src/ncvis.hpp:54). n_noise may be NULL (3 per epoch, src/ncvis.cpp:16-20)
+ * and is copied. n_threads is the number of sequential SGD chains (the reference's OpenMP
+ * threads, each seeded random_seed + thread id); 0 selects a GPU-sized chain count. M and
+ * ef_construction are accepted for signature compatibility; the search is exact. */
+int ncvis_create(size_t d, size_t n_threads, size_t n_neighbors, size_t M, size_t ef_construction,
+                 size_t random_seed, int n_epochs, int n_init_epochs, float a, float b, float alpha,
+                 float alpha_Q, const size_t *n_noise, int dist, ncvis_handle **out);
+
+void ncvis_destroy(ncvis_handle *h);
+
+/* Message of the last failed call on this handle ("" if none). Valid until the next call. */
+const char *ncvis_last_error(const ncvis_handle *h);
+
+/* Options (no reference counterpart):
+ *   "knn_precision"   NCVIS_PREC_*  (default NCVIS_PREC_FP16)
+ *   "atomic_updates"  1: Hogwild Y updates with float atomics (default for n_threads != 1),
+ *                     0: plain read-modify-write stores like the reference
+ *   "chains"          override the SGD chain count (0 = from n_threads) */
+int ncvis_set_option(ncvis_handle *h, const char *key, double value);
+
+/* ncvis::NCVis::fit_transform: X [N, D] and Y [N, d] are host, row-major float32. */
+int ncvis_fit_transform(ncvis_handle *h, const float *X, size_t N, size_t D, float *Y);
+
+/* Same with X and Y already in device memory (HBM). */
+int ncvis_fit_transform_device(ncvis_handle *h, const float *dX, size_t N, size_t D, float *dY, void *stream);
+
+/* Stage times of the last fit (milliseconds): [knn, symmetrize, init, optimize, total, knn_tiles,
+ * knn_rerank], the edge count E, the final Q and the effective chain count. */
+int ncvis_get_stats(const ncvis_handle *h, double *stage_ms7, int64_t *n_edges, float *final_Q, int64_t *chains);
+
+/* ---- stage entry points (device pointers) ---- */
+const char *ncvis_get_last_error(void);
+
+/* findKNN row format: d_inds [N, k] int32 / d_dists [N, k] float32, farthest first. */
+int ncvis_knn_device(const float *dX, size_t N, size_t D, size_t k, int dist, int precision, int32_t *d_inds,
+                     float *d_dists, void *stream);
+
+/* Symmetrized CSR: d_row_ptr [N+1] int64, d_col / d_dist capacity 2*N*k. *n_edges receives E. */
+int ncvis_symmetrize_device(const int32_t *d_inds, const float *d_dists, size_t N, size_t k, int64_t *d_row_ptr,
+                            int32_t *d_col, float *d_dist, int64_t *n_edges, void *stream);
+
+/* Random layout + n_init_epochs propagation epochs; n_threads = the reference's thread count
+ * whose static schedule splits the random draws. */
+int ncvis_init_embedding_device(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col, float alpha,
+                                int n_init_epochs, uint64_t seed, int64_t n_threads, float *dY, void *stream);
+
+/* SGD epochs. Q is host in/out; q_trace (host, optional) receives Q after each epoch. */
+int ncvis_optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t n_edges,
+                          float *dY, float *Q, int n_epochs, const size_t *n_noise, float a, float b, float alpha,
+                          float alpha_Q, uint64_t seed, int64_t n_threads, int atomic_updates, float *q_trace,
+                          void *stream);
+
+/* NCE loss over explicit negatives d_neg [E, n] int32. d_loss / d_gradQ: device double;
+ * d_gradY: device float [N, d], accumulated into (may be NULL). */
+int ncvis_nce_loss_device(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t n_edges,
+                          const int32_t *d_neg, int n, const float *dY, float Q, float a, float b, double *d_loss,
+                          float *d_gradY, double *d_gradQ, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NCVIS_AMD_H */

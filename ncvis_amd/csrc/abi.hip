@@ -1,0 +1,306 @@
+// C ABI of the NCVis HIP library: the handle API behind ncvis::NCVis (src/ncvis.hpp:34-93)
+// and the device-pointer stage entry points. Declarations and reference citations are in
+// include/ncvis_amd.h.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "ncvis_amd.h"
+
+using namespace ncvis_hip;
+
+struct ncvis_handle {
+    size_t d = 2, n_threads = 1, n_neighbors = 30, M = 16, ef_construction = 200, random_seed = 42;
+    int n_epochs = 50, n_init_epochs = 20;
+    float a = 1.f, b = 1.f, alpha = 1.f, alpha_Q = 1.f;
+    std::vector<uint64_t> n_noise;
+    int dist = DIST_L2;
+    int precision = PREC_FP16;
+    int atomic_updates = -1;  // -1: atomics unless a single chain
+    int64_t chains_override = 0;
+    std::string last_error;
+    hipStream_t stream = nullptr;
+    // stats of the last fit
+    double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t n_edges = 0;
+    float final_Q = 0.f;
+    int64_t chains_used = 0;
+};
+
+static thread_local std::string g_last_error;
+
+template <typename F>
+static int guarded(std::string &err, F &&f) {
+    try {
+        f();
+        err.clear();
+        return NCVIS_OK;
+    } catch (const HipError &e) {
+        err = e.what();
+        return NCVIS_EHIP;
+    } catch (const std::invalid_argument &e) {
+        err = e.what();
+        return NCVIS_EINVAL;
+    } catch (const std::exception &e) {
+        err = e.what();
+        return NCVIS_EINTERNAL;
+    } catch (...) {
+        err = "[ncvis_amd] unknown error";
+        return NCVIS_EINTERNAL;
+    }
+}
+
+// GPU-sized number of SGD chains: about 64 edges per chain, at most 2^18 chains.
+static int64_t auto_chains(int64_t E) { return std::max<int64_t>(1, std::min<int64_t>(E / 64, int64_t(1) << 18)); }
+
+static int64_t resolve_chains(const ncvis_handle *h, int64_t E) {
+    if (h->chains_override > 0) return h->chains_override;
+    if (h->n_threads > 0) return (int64_t)h->n_threads;
+    return auto_chains(E);
+}
+
+extern "C" {
+
+int ncvis_create(size_t d, size_t n_threads, size_t n_neighbors, size_t M, size_t ef_construction, size_t random_seed,
+                 int n_epochs, int n_init_epochs, float a, float b, float alpha, float alpha_Q, const size_t *n_noise,
+                 int dist, ncvis_handle **out) {
+    return guarded(g_last_error, [&] {
+        if (!out) throw std::invalid_argument("[ncvis_amd] ncvis_create: null output handle");
+        *out = nullptr;
+        if (dist < DIST_L2 || dist > DIST_CORRELATION)
+            throw std::invalid_argument("[ncvis::NCVis::buildKNN] Unrecognized distance type.");
+        if (d < 1) throw std::invalid_argument("[ncvis_amd] embedding dimension d must be >= 1");
+        if (n_epochs < 0 || n_init_epochs < 0) throw std::invalid_argument("[ncvis_amd] epoch counts must be >= 0");
+        auto *h = new ncvis_handle();
+        h->d = d;
+        h->n_threads = n_threads;
+        h->n_neighbors = n_neighbors;
+        h->M = M;
+        h->ef_construction = ef_construction;
+        h->random_seed = random_seed;
+        h->n_epochs = n_epochs;
+        h->n_init_epochs = n_init_epochs;
+        h->a = a;
+        h->b = b;
+        h->alpha = alpha;
+        h->alpha_Q = alpha_Q;
+        h->dist = dist;
+        h->n_noise.resize((size_t)n_epochs);
+        for (int i = 0; i < n_epochs; ++i) h->n_noise[i] = n_noise ? (uint64_t)n_noise[i] : 3;  // ncvis.cpp:16-20
+        for (uint64_t v : h->n_noise)
+            if (v > 0xFFFFFFFFull) {
+                delete h;
+                throw std::invalid_argument("[ncvis_amd] n_noise entry too large");
+            }
+        hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete h;
+            throw HipError(std::string("[ncvis_amd] cannot create a HIP stream: ") + hipGetErrorString(e));
+        }
+        *out = h;
+    });
+}
+
+void ncvis_destroy(ncvis_handle *h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char *ncvis_last_error(const ncvis_handle *h) { return h ? h->last_error.c_str() : g_last_error.c_str(); }
+
+const char *ncvis_get_last_error(void) { return g_last_error.c_str(); }
+
+int ncvis_set_option(ncvis_handle *h, const char *key, double value) {
+    if (!h) return NCVIS_EINVAL;
+    return guarded(h->last_error, [&] {
+        std::string k = key ? key : "";
+        if (k == "knn_precision") {
+            int v = (int)value;
+            if (v < PREC_FP32 || v > PREC_FP16) throw std::invalid_argument("[ncvis_amd] knn_precision must be 0, 1 or 2");
+            h->precision = v;
+        } else if (k == "atomic_updates") {
+            h->atomic_updates = (int)value;
+        } else if (k == "chains") {
+            if (value < 0) throw std::invalid_argument("[ncvis_amd] chains must be >= 0");
+            h->chains_override = (int64_t)value;
+        } else {
+            throw std::invalid_argument("[ncvis_amd] unknown option '" + k + "'");
+        }
+    });
+}
+
+int ncvis_fit_transform_device(ncvis_handle *h, const float *dX, size_t N, size_t D, float *dY, void *stream) {
+    if (!h) return NCVIS_EINVAL;
+    return guarded(h->last_error, [&] {
+        // ncvis.cpp:364-369
+        if (N == 0 || D == 0)
+            throw std::invalid_argument("[ncvis::NCVis::fit_transform] Dataset should have at least one element.");
+        if (dY == nullptr)
+            throw std::invalid_argument("[ncvis::NCVis::fit_transform] Null pointer provided for output.");
+        if (N < 2) throw std::invalid_argument("[ncvis::NCVis::fit_transform] At least two points are required.");
+        if (N > (size_t)INT32_MAX - 1024) throw std::invalid_argument("[ncvis_amd] more than 2^31 points");
+        hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+        // ncvis.cpp:381-382
+        size_t kk = (h->n_neighbors < N - 1) ? h->n_neighbors : (N - 1);
+        kk = (kk > 0) ? kk : 1;
+        const int k = (int)kk;
+        const int64_t n = (int64_t)N;
+        hipEvent_t ev[5];
+        for (auto &e : ev) NCVIS_HIP_CHECK(hipEventCreate(&e));
+        struct EvGuard {
+            hipEvent_t *e;
+            ~EvGuard() {
+                for (int i = 0; i < 5; ++i) (void)hipEventDestroy(e[i]);
+            }
+        } evg{ev};
+        NCVIS_HIP_CHECK(hipEventRecord(ev[0], st));
+        DevBuf<int32_t> inds((size_t)n * k);
+        DevBuf<float> dists((size_t)n * k);
+        KnnStats ks;
+        knn_run(dX, n, (int)D, k, h->dist, h->precision, inds.p, dists.p, st, &ks);
+        NCVIS_HIP_CHECK(hipEventRecord(ev[1], st));
+        DevBuf<int64_t> row_ptr((size_t)n + 1);
+        DevBuf<int32_t> col((size_t)2 * n * k);
+        DevBuf<float> cdist((size_t)2 * n * k);
+        const int64_t E = symmetrize_run(inds.p, dists.p, n, k, row_ptr.p, col.p, cdist.p, st);
+        inds.release();
+        dists.release();
+        cdist.release();
+        NCVIS_HIP_CHECK(hipEventRecord(ev[2], st));
+        const int64_t T = resolve_chains(h, E);
+        const float init_alpha = (float)(1. / k);  // ncvis.cpp:418
+        init_embedding_run(n, (int)h->d, row_ptr.p, col.p, init_alpha, h->n_init_epochs, h->random_seed, T, dY, st);
+        NCVIS_HIP_CHECK(hipEventRecord(ev[3], st));
+        SgdParams p;
+        p.a = h->a;
+        p.b = h->b;
+        p.alpha = h->alpha;
+        p.alpha_Q = h->alpha_Q;
+        p.seed = h->random_seed;
+        p.n_epochs = h->n_epochs;
+        p.n_noise = h->n_noise.data();
+        p.chains = T;
+        p.atomic_updates = h->atomic_updates >= 0 ? h->atomic_updates : (T > 1 ? 1 : 0);
+        float Q = 0.f;  // ncvis.cpp:417
+        optimize_run(n, (int)h->d, row_ptr.p, col.p, E, dY, &Q, p, st, nullptr);
+        NCVIS_HIP_CHECK(hipEventRecord(ev[4], st));
+        NCVIS_HIP_CHECK(hipEventSynchronize(ev[4]));
+        float ms[4];
+        for (int i = 0; i < 4; ++i) NCVIS_HIP_CHECK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        for (int i = 0; i < 4; ++i) h->stage_ms[i] = ms[i];
+        h->stage_ms[4] = ms[0] + ms[1] + ms[2] + ms[3];
+        h->stage_ms[5] = ks.tile_ms;
+        h->stage_ms[6] = ks.rerank_ms;
+        h->n_edges = E;
+        h->final_Q = Q;
+        h->chains_used = T;
+    });
+}
+
+int ncvis_fit_transform(ncvis_handle *h, const float *X, size_t N, size_t D, float *Y) {
+    if (!h) return NCVIS_EINVAL;
+    int rc = guarded(h->last_error, [&] {
+        if (N == 0 || D == 0)
+            throw std::invalid_argument("[ncvis::NCVis::fit_transform] Dataset should have at least one element.");
+        if (Y == nullptr)
+            throw std::invalid_argument("[ncvis::NCVis::fit_transform] Null pointer provided for output.");
+        if (X == nullptr) throw std::invalid_argument("[ncvis_amd] null input pointer");
+    });
+    if (rc) return rc;
+    DevBuf<float> dX, dY;
+    rc = guarded(h->last_error, [&] {
+        dX.alloc(N * D);
+        dY.alloc(N * h->d);
+        NCVIS_HIP_CHECK(hipMemcpyAsync(dX.p, X, sizeof(float) * N * D, hipMemcpyHostToDevice, h->stream));
+    });
+    if (rc) return rc;
+    rc = ncvis_fit_transform_device(h, dX.p, N, D, dY.p, h->stream);
+    if (rc) return rc;
+    return guarded(h->last_error, [&] {
+        NCVIS_HIP_CHECK(hipMemcpyAsync(Y, dY.p, sizeof(float) * N * h->d, hipMemcpyDeviceToHost, h->stream));
+        NCVIS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int ncvis_get_stats(const ncvis_handle *h, double *stage_ms7, int64_t *n_edges, float *final_Q, int64_t *chains) {
+    if (!h) return NCVIS_EINVAL;
+    if (stage_ms7)
+        for (int i = 0; i < 7; ++i) stage_ms7[i] = h->stage_ms[i];
+    if (n_edges) *n_edges = h->n_edges;
+    if (final_Q) *final_Q = h->final_Q;
+    if (chains) *chains = h->chains_used;
+    return NCVIS_OK;
+}
+
+int ncvis_knn_device(const float *dX, size_t N, size_t D, size_t k, int dist, int precision, int32_t *d_inds,
+                     float *d_dists, void *stream) {
+    return guarded(g_last_error, [&] {
+        if (!dX || !d_inds || !d_dists) throw std::invalid_argument("[ncvis_amd] knn: null pointer");
+        if (dist < DIST_L2 || dist > DIST_CORRELATION) throw std::invalid_argument("[ncvis_amd] knn: bad distance");
+        if (N < 2 || D < 1 || k < 1 || k > N - 1) throw std::invalid_argument("[ncvis_amd] knn: need 1 <= k <= N-1");
+        knn_run(dX, (int64_t)N, (int)D, (int)k, dist, precision, d_inds, d_dists, (hipStream_t)stream);
+    });
+}
+
+int ncvis_symmetrize_device(const int32_t *d_inds, const float *d_dists, size_t N, size_t k, int64_t *d_row_ptr,
+                            int32_t *d_col, float *d_dist, int64_t *n_edges, void *stream) {
+    return guarded(g_last_error, [&] {
+        if (!d_inds || !d_dists || !d_row_ptr || !d_col || !d_dist)
+            throw std::invalid_argument("[ncvis_amd] symmetrize: null pointer");
+        if (N < 1 || k < 1) throw std::invalid_argument("[ncvis_amd] symmetrize: empty table");
+        int64_t E = symmetrize_run(d_inds, d_dists, (int64_t)N, (int)k, d_row_ptr, d_col, d_dist, (hipStream_t)stream);
+        if (n_edges) *n_edges = E;
+    });
+}
+
+int ncvis_init_embedding_device(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col, float alpha,
+                                int n_init_epochs, uint64_t seed, int64_t n_threads, float *dY, void *stream) {
+    return guarded(g_last_error, [&] {
+        if (!d_row_ptr || !dY) throw std::invalid_argument("[ncvis_amd] init: null pointer");
+        init_embedding_run((int64_t)N, (int)d, d_row_ptr, d_col, alpha, n_init_epochs, seed,
+                           n_threads < 1 ? 1 : n_threads, dY, (hipStream_t)stream);
+    });
+}
+
+int ncvis_optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t n_edges,
+                          float *dY, float *Q, int n_epochs, const size_t *n_noise, float a, float b, float alpha,
+                          float alpha_Q, uint64_t seed, int64_t n_threads, int atomic_updates, float *q_trace,
+                          void *stream) {
+    return guarded(g_last_error, [&] {
+        if (!d_row_ptr || !d_col || !dY || !Q) throw std::invalid_argument("[ncvis_amd] optimize: null pointer");
+        if (n_epochs > 0 && !n_noise) throw std::invalid_argument("[ncvis_amd] optimize: null n_noise");
+        std::vector<uint64_t> nn((size_t)std::max(n_epochs, 0));
+        for (int i = 0; i < n_epochs; ++i) {
+            if ((uint64_t)n_noise[i] > 0xFFFFFFFFull) throw std::invalid_argument("[ncvis_amd] n_noise too large");
+            nn[i] = (uint64_t)n_noise[i];
+        }
+        SgdParams p;
+        p.a = a;
+        p.b = b;
+        p.alpha = alpha;
+        p.alpha_Q = alpha_Q;
+        p.seed = seed;
+        p.n_epochs = n_epochs;
+        p.n_noise = nn.data();
+        p.chains = n_threads < 1 ? auto_chains(n_edges) : n_threads;
+        p.atomic_updates = atomic_updates;
+        optimize_run((int64_t)N, (int)d, d_row_ptr, d_col, n_edges, dY, Q, p, (hipStream_t)stream, q_trace);
+    });
+}
+
+int ncvis_nce_loss_device(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t n_edges,
+                          const int32_t *d_neg, int n, const float *dY, float Q, float a, float b, double *d_loss,
+                          float *d_gradY, double *d_gradQ, void *stream) {
+    return guarded(g_last_error, [&] {
+        if (!d_row_ptr || !d_col || !d_neg || !dY || !d_loss)
+            throw std::invalid_argument("[ncvis_amd] nce_loss: null pointer");
+        nce_loss_run((int64_t)N, (int)d, d_row_ptr, d_col, n_edges, d_neg, n, dY, Q, a, b, d_loss, d_gradY, d_gradQ,
+                     (hipStream_t)stream);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,458 @@
+// Layout initialisation, NCE SGD epochs and the explicit NCE loss / adjoint on gfx950.
+//
+// Initialisation (src/ncvis.cpp:184-273):
+//   k_init_random  Y[i] = uniform_real<float>(0,1) drawn from pcg64(seed + t), where t is the
+//                  thread that owns element i under libgomp's static schedule of the N*d loop
+//                  (ncvis.cpp:198-201); a GPU thread jumps the stream to its first element.
+//   k_spmm_rows    Y_new[i] = sum_{edges (i, j) in row order} alpha * Y_old[j], row-owned
+//                  (race-free), uncontracted like the reference's loop (ncvis.cpp:206-218)
+//   k_colsum / k_colvar / k_finalize / k_normalize
+//                  per-dimension mean and population std (FP64 partial sums, deterministic)
+//                  and standardisation (ncvis.cpp:220-260)
+// Epochs (src/ncvis.cpp:275-352):
+//   k_sgd_epoch    T chains, chain t = the reference's OpenMP thread t: pcg64(seed + t), the
+//                  static edge chunk [s0, e0) of thread t, a private Q copy, sequential samples.
+//                  Chains run concurrently Hogwild-style; T = 1 is the reference at one thread.
+//   k_q_reduce     Q = mean of the chains' Q copies (ncvis.cpp:342-349), FP64 sum.
+// NCE loss (the objective the update of ncvis.cpp:304-338 descends):
+//   k_nce_loss     per edge: positive + n explicit negatives; loss and dL/dQ as FP64 partial
+//                  sums, dL/dY scattered with FP32 atomics.
+#include <cfloat>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace ncvis_hip {
+
+// ------------------------------------------------------------------------ init
+__global__ void k_init_random(float *__restrict__ Y, int64_t Nd, int64_t T, uint64_t seed) {
+    constexpr int RUN = 64;
+    int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * RUN;
+    if (i0 >= Nd) return;
+    int64_t i1 = i0 + RUN < Nd ? i0 + RUN : Nd;
+    // owner of element i0 under the static schedule
+    int64_t q = Nd / T, r = Nd % T, t;
+    if (i0 < r * (q + 1))
+        t = i0 / (q + 1);
+    else
+        t = r + (i0 - r * (q + 1)) / q;
+    int64_t s0, e0;
+    static_chunk(Nd, T, t, s0, e0);
+    U128 g = pcg_advance(pcg_seed(seed + (uint64_t)t), (uint64_t)(i0 - s0));
+    for (int64_t i = i0; i < i1; ++i) {
+        while (i >= e0) {
+            ++t;
+            static_chunk(Nd, T, t, s0, e0);
+            g = pcg_seed(seed + (uint64_t)t);
+        }
+        Y[i] = uniform_unit_float(g);
+    }
+}
+
+__global__ void k_spmm_rows(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col, int64_t N, int d,
+                            float alpha, const float *__restrict__ Yo, float *__restrict__ Yn) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int64_t b = row_ptr[i], e = row_ptr[i + 1];
+    for (int c = 0; c < d; ++c) {
+        float acc = 0.f;
+        for (int64_t p = b; p < e; ++p) acc = __fadd_rn(acc, __fmul_rn(alpha, Yo[(int64_t)col[p] * d + c]));
+        Yn[i * d + c] = acc;
+    }
+}
+
+// per-block FP64 column sums of Y (mode 0) or of (Y - mean)^2 (mode 1)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_colstat(const float *__restrict__ Y, int64_t N, int d,
+                                                 const float *__restrict__ mean, double *__restrict__ partial) {
+    __shared__ double red[256];
+    for (int c = 0; c < d; ++c) {
+        double s = 0.0;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+            float v = Y[i * d + c];
+            if (MODE == 1) {
+                float dl = __fsub_rn(v, mean[c]);
+                v = __fmul_rn(dl, dl);
+            }
+            s += (double)v;
+        }
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * d + c] = red[0];
+        __syncthreads();
+    }
+}
+
+// MODE 0: mean[c] = sum / N; MODE 1: sigma[c] = sqrtf(sum / N)
+template <int MODE>
+__global__ void k_finalize(const double *__restrict__ partial, int nb, int d, int64_t N, float *__restrict__ out) {
+    int c = threadIdx.x;
+    if (c >= d) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += partial[(int64_t)b * d + c];
+    float v = (float)(s / (double)N);
+    out[c] = (MODE == 0) ? v : sqrtf(v);
+}
+
+__global__ void k_normalize(float *__restrict__ Y, int64_t N, int d, const float *__restrict__ mean,
+                            const float *__restrict__ sigma) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N * d) return;
+    int c = (int)(i % d);
+    Y[i] = __fsub_rn(Y[i], mean[c]) / sigma[c];
+}
+
+void init_embedding_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, float alpha,
+                        int n_init_epochs, uint64_t seed, int64_t chains, float *dY, hipStream_t st) {
+    if (N < 1 || d < 1 || d > 1024) throw HipError("[ncvis_amd] init: invalid sizes");
+    const int64_t Nd = N * d;
+    const int64_t T = chains < 1 ? 1 : chains;
+    k_init_random<<<cdiv(cdiv(Nd, 64), 256), 256, 0, st>>>(dY, Nd, T, seed);
+    NCVIS_LAUNCH_CHECK();
+    if (n_init_epochs <= 0) return;
+    DevBuf<float> y2((size_t)Nd);
+    const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, cdiv(N, 256)));
+    DevBuf<double> partial((size_t)nb * d);
+    DevBuf<float> stats((size_t)2 * d);
+    float *Ys[2] = {dY, y2.p};
+    for (int ep = 0; ep < n_init_epochs; ++ep) {
+        const float *Yo = Ys[ep % 2];
+        float *Yn = Ys[(ep + 1) % 2];
+        k_spmm_rows<<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, d_col, N, d, alpha, Yo, Yn);
+        NCVIS_LAUNCH_CHECK();
+        k_colstat<0><<<nb, 256, 0, st>>>(Yn, N, d, nullptr, partial.p);
+        k_finalize<0><<<1, 1024, 0, st>>>(partial.p, nb, d, N, stats.p);
+        k_colstat<1><<<nb, 256, 0, st>>>(Yn, N, d, stats.p, partial.p);
+        k_finalize<1><<<1, 1024, 0, st>>>(partial.p, nb, d, N, stats.p + d);
+        k_normalize<<<cdiv(Nd, 256), 256, 0, st>>>(Yn, N, d, stats.p, stats.p + d);
+        NCVIS_LAUNCH_CHECK();
+    }
+    if (n_init_epochs % 2)
+        NCVIS_HIP_CHECK(hipMemcpyAsync(dY, y2.p, sizeof(float) * Nd, hipMemcpyDeviceToDevice, st));
+    NCVIS_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// ------------------------------------------------------------------------ SGD
+struct SampleConst {
+    float a, b, alpha_Q, step;
+    float noise_f;     // (float)cur_noise
+    uint32_t noise;    // cur_noise
+};
+
+// The sample body of ncvis.cpp:304-339 on a register copy of the two rows. Returns the weight
+// applied; updates yi / yo in place and the chain's Q copy.
+template <int DIM>
+__device__ __forceinline__ void sgd_math(float (&yi)[DIM], float (&yo)[DIM], bool positive, const SampleConst &c,
+                                         float &Qc, float (&dx)[DIM]) {
+    float d2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+        float t = __fsub_rn(yi[k], yo[k]);
+        d2 = __fadd_rn(d2, __fmul_rn(t, t));
+    }
+    float Ph = 1.f / (1.f + c.a * powf(d2, c.b));
+    float w = 1.f;
+    if (c.noise != 0) {
+        w = Ph / (c.noise_f * expf(Qc));
+        if (positive)
+            w = 1.f / (1.f + w);
+        else
+            w = -1.f / (1.f + 1.f / w);
+        Qc = __fsub_rn(Qc, __fmul_rn(w, c.alpha_Q));
+        w = 2.f * w * Ph * c.a * c.b * powf(d2, c.b - 1.f);
+    }
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+        float v = __fmul_rn(__fmul_rn(__fsub_rn(yo[k], yi[k]), w), c.step);
+        if (v > 4.f)
+            v = 4.f;
+        else if (v < -4.f)
+            v = -4.f;
+        dx[k] = v;
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void sgd_sample(float *__restrict__ Y, int64_t id, int64_t other, bool positive,
+                                           const SampleConst &c, float &Qc, int atomic_mode) {
+    float yi[DIM], yo[DIM], dx[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+        yi[k] = Y[id * DIM + k];
+        yo[k] = Y[other * DIM + k];
+    }
+    sgd_math<DIM>(yi, yo, positive, c, Qc, dx);
+    if (atomic_mode) {
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            __hip_atomic_fetch_add(&Y[id * DIM + k], dx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&Y[other * DIM + k], -dx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else {
+        // Y[id] += dx; Y[other] -= dx, in the reference's order (other may equal id)
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            float ni = __fadd_rn(yi[k], dx[k]);
+            Y[id * DIM + k] = ni;
+            float cur = (other == id) ? ni : yo[k];
+            Y[other * DIM + k] = __fsub_rn(cur, dx[k]);
+        }
+    }
+}
+
+// Generic-d variant working directly on memory (d not known at compile time).
+__device__ inline void sgd_sample_rt(float *__restrict__ Y, int d, int64_t id, int64_t other, bool positive,
+                                     const SampleConst &c, float &Qc, int atomic_mode) {
+    float d2 = 0.f;
+    for (int k = 0; k < d; ++k) {
+        float t = __fsub_rn(Y[id * d + k], Y[other * d + k]);
+        d2 = __fadd_rn(d2, __fmul_rn(t, t));
+    }
+    float Ph = 1.f / (1.f + c.a * powf(d2, c.b));
+    float w = 1.f;
+    if (c.noise != 0) {
+        w = Ph / (c.noise_f * expf(Qc));
+        if (positive)
+            w = 1.f / (1.f + w);
+        else
+            w = -1.f / (1.f + 1.f / w);
+        Qc = __fsub_rn(Qc, __fmul_rn(w, c.alpha_Q));
+        w = 2.f * w * Ph * c.a * c.b * powf(d2, c.b - 1.f);
+    }
+    for (int k = 0; k < d; ++k) {
+        float v = __fmul_rn(__fmul_rn(__fsub_rn(Y[other * d + k], Y[id * d + k]), w), c.step);
+        if (v > 4.f)
+            v = 4.f;
+        else if (v < -4.f)
+            v = -4.f;
+        if (atomic_mode) {
+            __hip_atomic_fetch_add(&Y[id * d + k], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&Y[other * d + k], -v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            Y[id * d + k] = __fadd_rn(Y[id * d + k], v);
+            Y[other * d + k] = __fsub_rn(Y[other * d + k], v);
+        }
+    }
+}
+
+// chain_row[t] = row holding edge s0 of chain t (row_ptr[r] <= s0 < row_ptr[r+1])
+__global__ void k_chain_rows(const int64_t *__restrict__ row_ptr, int64_t N, int64_t E, int64_t T,
+                             int64_t *__restrict__ chain_row, U128 *__restrict__ rng, uint64_t seed) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    int64_t s0, e0;
+    static_chunk(E, T, t, s0, e0);
+    int64_t lo = 0, hi = N;  // largest r with row_ptr[r] <= s0
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) / 2;
+        if (row_ptr[mid] <= s0)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    while (lo < N - 1 && row_ptr[lo + 1] <= s0) ++lo;
+    chain_row[t] = lo;
+    rng[t] = pcg_seed(seed + (uint64_t)t);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_sgd_epoch(float *__restrict__ Y, int d_rt, const int64_t *__restrict__ row_ptr,
+                                                   const int32_t *__restrict__ col, int64_t E, int64_t T,
+                                                   const int64_t *__restrict__ chain_row, U128 *__restrict__ rng,
+                                                   const float *__restrict__ Qdev, float *__restrict__ qpart,
+                                                   SampleConst c, uint64_t N, int atomic_mode) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    int64_t s0, e0;
+    static_chunk(E, T, t, s0, e0);
+    float Qc = *Qdev;
+    U128 g = rng[t];
+    int64_t r = chain_row[t];
+    int64_t rend = row_ptr[r + 1];
+    for (int64_t e = s0; e < e0; ++e) {
+        while (e >= rend) {
+            ++r;
+            rend = row_ptr[r + 1];
+        }
+        const int64_t id = r;
+        for (uint32_t j = 0; j <= c.noise; ++j) {
+            int64_t other;
+            if (j == 0) {
+                other = col[e];
+            } else {
+                do {
+                    other = (int64_t)uniform_index(g, N);
+                } while (other == id);
+            }
+            if constexpr (DIM > 0)
+                sgd_sample<DIM>(Y, id, other, j == 0, c, Qc, atomic_mode);
+            else
+                sgd_sample_rt(Y, d_rt, id, other, j == 0, c, Qc, atomic_mode);
+        }
+    }
+    rng[t] = g;
+    qpart[t] = Qc;
+}
+
+__global__ __launch_bounds__(256) void k_q_reduce(const float *__restrict__ qpart, int64_t T, float *__restrict__ Qdev,
+                                                  float *__restrict__ trace, int epoch) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int64_t t = threadIdx.x; t < T; t += 256) s += (double)qpart[t];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float q = (T == 1) ? qpart[0] : (float)(red[0] / (double)T);
+        *Qdev = q;
+        trace[epoch] = q;
+    }
+}
+
+static void launch_epoch(int d, dim3 grid, float *Y, const int64_t *row_ptr, const int32_t *col, int64_t E,
+                         int64_t T, const int64_t *chain_row, U128 *rng, const float *Qdev, float *qpart,
+                         const SampleConst &c, uint64_t N, int atomic_mode, hipStream_t st) {
+    switch (d) {
+        case 1: k_sgd_epoch<1><<<grid, 256, 0, st>>>(Y, d, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, atomic_mode); break;
+        case 2: k_sgd_epoch<2><<<grid, 256, 0, st>>>(Y, d, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, atomic_mode); break;
+        case 3: k_sgd_epoch<3><<<grid, 256, 0, st>>>(Y, d, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, atomic_mode); break;
+        case 4: k_sgd_epoch<4><<<grid, 256, 0, st>>>(Y, d, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, atomic_mode); break;
+        default: k_sgd_epoch<0><<<grid, 256, 0, st>>>(Y, d, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, atomic_mode); break;
+    }
+}
+
+void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t E, float *dY, float *Q,
+                  const SgdParams &p, hipStream_t st, float *q_trace) {
+    if (N < 2 || d < 1) throw HipError("[ncvis_amd] optimize: invalid sizes");
+    const int64_t T = p.chains < 1 ? 1 : p.chains;
+    DevBuf<int64_t> chain_row((size_t)T);
+    DevBuf<U128> rng((size_t)T);
+    DevBuf<float> qpart((size_t)T);
+    DevBuf<float> qdev(1 + (size_t)std::max(p.n_epochs, 1));
+    k_chain_rows<<<cdiv(T, 256), 256, 0, st>>>(d_row_ptr, N, E, T, chain_row.p, rng.p, p.seed);
+    NCVIS_LAUNCH_CHECK();
+    NCVIS_HIP_CHECK(hipMemcpyAsync(qdev.p, Q, sizeof(float), hipMemcpyHostToDevice, st));
+    dim3 grid((unsigned)cdiv(T, 256));
+    for (int epoch = 0; epoch < p.n_epochs; ++epoch) {
+        SampleConst c;
+        c.a = p.a;
+        c.b = p.b;
+        c.alpha_Q = p.alpha_Q;
+        float fe = (float)epoch / (float)p.n_epochs;
+        c.step = p.alpha * (1.f - fe * fe);
+        c.noise = (uint32_t)p.n_noise[epoch];
+        c.noise_f = (float)p.n_noise[epoch];
+        launch_epoch(d, grid, dY, d_row_ptr, d_col, E, T, chain_row.p, rng.p, qdev.p, qpart.p, c, (uint64_t)N,
+                     p.atomic_updates, st);
+        NCVIS_LAUNCH_CHECK();
+        k_q_reduce<<<1, 256, 0, st>>>(qpart.p, T, qdev.p, qdev.p + 1, epoch);
+        NCVIS_LAUNCH_CHECK();
+    }
+    NCVIS_HIP_CHECK(hipMemcpyAsync(Q, qdev.p, sizeof(float), hipMemcpyDeviceToHost, st));
+    if (q_trace && p.n_epochs > 0)
+        NCVIS_HIP_CHECK(
+            hipMemcpyAsync(q_trace, qdev.p + 1, sizeof(float) * p.n_epochs, hipMemcpyDeviceToHost, st));
+    NCVIS_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// ------------------------------------------------------------------------ NCE loss
+__global__ void k_edge_src(const int64_t *__restrict__ row_ptr, int64_t N, int32_t *__restrict__ src) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) src[e] = (int32_t)i;
+}
+
+// s = log Ph - Q - log n, Ph = 1/(1 + a d2^b); positive term softplus(-s), negative softplus(s).
+// dL/ds: positive -(1 - sigmoid(s)), negative sigmoid(s); ds/dd2 = -a b d2^(b-1) Ph.
+__global__ __launch_bounds__(256) void k_nce_loss(const float *__restrict__ Y, int d, const int32_t *__restrict__ src,
+                                                  const int32_t *__restrict__ col, int64_t E,
+                                                  const int32_t *__restrict__ neg, int n, float Q, float a, float b,
+                                                  double *__restrict__ partial, float *__restrict__ gY) {
+    __shared__ double red_l[256], red_q[256];
+    double L = 0.0, gQ = 0.0;
+    const float logn = logf((float)n);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = src[e];
+        for (int m = 0; m <= n; ++m) {
+            const int64_t o = (m == 0) ? col[e] : neg[e * n + (m - 1)];
+            float d2 = 0.f;
+            for (int c = 0; c < d; ++c) {
+                float t = Y[i * d + c] - Y[o * d + c];
+                d2 = fmaf(t, t, d2);
+            }
+            float pw = powf(d2, b);
+            float logPh = -log1pf(a * pw);
+            float s = logPh - Q - logn;
+            float sig = 1.f / (1.f + expf(-s));
+            float g;
+            if (m == 0) {
+                L += (double)((s > 0.f) ? log1pf(expf(-s)) : (-s + log1pf(expf(s))));
+                g = -(1.f - sig);
+            } else {
+                L += (double)((s > 0.f) ? (s + log1pf(expf(-s))) : log1pf(expf(s)));
+                g = sig;
+            }
+            gQ -= (double)g;
+            if (gY && d2 > 0.f) {
+                float dsdd2 = -a * b * (pw / d2) / (1.f + a * pw);
+                float coef = 2.f * g * dsdd2;
+                for (int c = 0; c < d; ++c) {
+                    float gi = coef * (Y[i * d + c] - Y[o * d + c]);
+                    atomicAdd(&gY[i * d + c], gi);
+                    atomicAdd(&gY[o * d + c], -gi);
+                }
+            }
+        }
+    }
+    red_l[threadIdx.x] = L;
+    red_q[threadIdx.x] = gQ;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            red_l[threadIdx.x] += red_l[threadIdx.x + w];
+            red_q[threadIdx.x] += red_q[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = red_l[0];
+        partial[2 * blockIdx.x + 1] = red_q[0];
+    }
+}
+
+__global__ void k_nce_finalize(const double *__restrict__ partial, int nb, double *__restrict__ loss,
+                               double *__restrict__ gQ) {
+    if (threadIdx.x != 0) return;
+    double l = 0, q = 0;
+    for (int b = 0; b < nb; ++b) {
+        l += partial[2 * b];
+        q += partial[2 * b + 1];
+    }
+    *loss = l;
+    if (gQ) *gQ = q;
+}
+
+void nce_loss_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t E, const int32_t *d_neg,
+                  int n, const float *dY_in, float Q, float a, float b, double *d_loss, float *d_gradY,
+                  double *d_gradQ, hipStream_t st) {
+    if (N < 1 || d < 1 || n < 1) throw HipError("[ncvis_amd] nce_loss: needs N >= 1, d >= 1, n >= 1");
+    DevBuf<int32_t> src((size_t)std::max<int64_t>(E, 1));
+    k_edge_src<<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, N, src.p);
+    NCVIS_LAUNCH_CHECK();
+    const int nb = (int)std::min<int64_t>(2048, std::max<int64_t>(1, cdiv(E, 256)));
+    DevBuf<double> partial((size_t)2 * nb);
+    k_nce_loss<<<nb, 256, 0, st>>>(dY_in, d, src.p, d_col, E, d_neg, n, Q, a, b, partial.p, d_gradY);
+    NCVIS_LAUNCH_CHECK();
+    k_nce_finalize<<<1, 64, 0, st>>>(partial.p, nb, d_loss, d_gradQ);
+    NCVIS_LAUNCH_CHECK();
+    NCVIS_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace ncvis_hip

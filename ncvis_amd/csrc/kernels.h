@@ -1,0 +1,50 @@
+// Internal host-side launchers of the NCVis HIP kernels (not part of the C ABI).
+#pragma once
+
+#include "common.h"
+
+namespace ncvis_hip {
+
+// ---- kNN (knn.hip) ---------------------------------------------------------------------
+// Exact k-nearest-neighbour table in the reference's findKNN row format: for each row the k+1
+// smallest (distance, label) pairs are found, the smallest is dropped and the remaining k are
+// written farthest first. `prec` selects the candidate GEMM precision; the final distances are
+// always exact FP32 (squared L2, or 1 - <x,y> for the inner-product family).
+struct KnnStats {
+    double tile_ms = 0;    // candidate GEMM + top-k kernel time
+    double rerank_ms = 0;  // exact rerank time
+    int used_mfma = 0;     // 1 if the MFMA tile kernel ran, 0 if the generic kernel ran
+};
+void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32_t *d_inds, float *d_dists,
+             hipStream_t st, KnnStats *stats = nullptr);
+
+// ---- symmetrize (symmetrize.hip) ---------------------------------------------------------
+// KNNTable::symmetrize at one thread, bit-exact, then flattened to CSR (= build_edges order).
+// d_col / d_dist must hold 2*N*k entries; d_row_ptr N+1. Returns E.
+int64_t symmetrize_run(const int32_t *d_inds, const float *d_dists, int64_t N, int k, int64_t *d_row_ptr,
+                       int32_t *d_col, float *d_dist, hipStream_t st);
+
+// ---- embedding (embed.hip) ---------------------------------------------------------------
+struct SgdParams {
+    float a, b, alpha, alpha_Q;
+    uint64_t seed;
+    int n_epochs;
+    const uint64_t *n_noise;  // host array [n_epochs]
+    int64_t chains;           // T: number of sequential SGD chains (the reference's n_threads)
+    int atomic_updates;       // 1: Y updates with atomics (Hogwild without lost updates)
+};
+
+void init_embedding_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, float alpha,
+                        int n_init_epochs, uint64_t seed, int64_t chains, float *dY, hipStream_t st);
+
+// Runs all epochs; Q (host in/out). `q_trace` (optional, host [n_epochs]) receives Q per epoch.
+void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t E, float *dY,
+                  float *Q, const SgdParams &p, hipStream_t st, float *q_trace = nullptr);
+
+// NCE loss and adjoint over an explicit negative table neg[E, n] (int32). dY (float, [N,d],
+// accumulated) and dQ / loss (double) are device pointers; dY may be null for loss only.
+void nce_loss_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t E,
+                  const int32_t *d_neg, int n, const float *dY_in, float Q, float a, float b, double *d_loss,
+                  float *d_gradY, double *d_gradQ, hipStream_t st);
+
+}  // namespace ncvis_hip

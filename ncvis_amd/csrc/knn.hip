@@ -1,0 +1,562 @@
+// Exact all-pairs kNN for the NCVis fit path on gfx950.
+//
+// Replaces buildKNN + findKNN (src/ncvis.cpp:72-153, hnswlib HNSW) with an exact search that
+// keeps findKNN's output contract: per row, the k+1 smallest (distance, label) pairs, the single
+// smallest dropped (normally the point itself), the remaining k stored farthest first
+// (src/ncvis.cpp:142-147). Distances follow hnswlib's spaces: squared L2 for euclidean,
+// 1 - <x, y> for inner_product / cosine / correlation, on rows preprocessed as in
+// src/ncvis.cpp:32-70.
+//
+// Pipeline:
+//   k_preprocess  - cosine/correlation row preprocessing (one thread per row, FP32)
+//   k_pack        - candidate matrix in the GEMM precision (FP32, BF16 or FP16), zero padded to
+//                   Dpad = 32*DK columns, plus per-row score offsets cn (||x||^2 for L2, 0 for IP,
+//                   +inf for padding rows)
+//   k_knn_tiles   - MFMA tiles: a workgroup owns 4 waves x 32*QB queries whose B fragments live
+//                   in registers; candidate rows stream through double-buffered LDS stages; every
+//                   32x32 MFMA block is scored (cn + beta * <q, c>) and filtered into per-lane
+//                   sorted top-KL lists (two half-lists per query, one per lane half)
+//   k_knn_rerank  - exact FP32 distances for the 2*KL candidates of each query, rank by
+//                   (distance, label), drop rank 0, write ranks k..1 (farthest first)
+//   k_knn_generic - any D / any k: one thread per query, exact FP32 scan with a max-heap
+#include <cfloat>
+#include <climits>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace ncvis_hip {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+// ------------------------------------------------------------------------ preprocess
+// src/ncvis.cpp:32-70; sums are sequential and uncontracted like the reference's loops.
+__global__ void k_preprocess(const float *__restrict__ X, int64_t N, int D, int dist, float *__restrict__ Xp) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float *x = X + i * D;
+    float *o = Xp + i * D;
+    if (dist == DIST_CORRELATION) {
+        float M = 0.f;
+        for (int j = 0; j < D; ++j) M = __fadd_rn(M, x[j]);
+        M = M / (float)D;
+        for (int j = 0; j < D; ++j) o[j] = __fsub_rn(x[j], M);
+    } else {
+        for (int j = 0; j < D; ++j) o[j] = x[j];
+    }
+    if (dist == DIST_CORRELATION || dist == DIST_COSINE) {
+        float s = 0.f;
+        for (int j = 0; j < D; ++j) s = __fadd_rn(s, __fmul_rn(o[j], o[j]));
+        if (s != 0.f) {
+            float r = sqrtf(s);
+            for (int j = 0; j < D; ++j) o[j] = o[j] / r;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------ packing
+__device__ inline uint16_t f32_to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Element k of a padded row as stored for MFMA. 16-bit precisions keep the natural order
+// (lane half h of a 32x32x16 step reads k = 16 s + 8 h .. +7, contiguous). FP32 (32x32x2:
+// lane half h reads k = 2 s + h) is stored permuted, position h*(Dpad/2) + s, so a lane's
+// operands are contiguous too. A and B use the same permutation, so dot products are unchanged.
+template <int P>
+__global__ void k_pack(const float *__restrict__ Xp, int64_t N, int64_t Npad, int D, int Dpad, int is_l2,
+                       void *__restrict__ Xh, float *__restrict__ cn) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Npad) return;
+    float s = 0.f;
+    if (P == PREC_FP32) {
+        float *o = (float *)Xh + i * Dpad;
+        const int half = Dpad / 2;
+        for (int p = 0; p < Dpad; ++p) {
+            int h = p / half, st = p - h * half;
+            int kk = 2 * st + h;
+            float v = (i < N && kk < D) ? Xp[i * D + kk] : 0.f;
+            o[p] = v;
+            s = __fadd_rn(s, __fmul_rn(v, v));
+        }
+    } else {
+        uint16_t *o = (uint16_t *)Xh + i * Dpad;
+        for (int kk = 0; kk < Dpad; ++kk) {
+            float v = (i < N && kk < D) ? Xp[i * D + kk] : 0.f;
+            float r;
+            if (P == PREC_BF16) {
+                uint16_t b = f32_to_bf16_rne(v);
+                o[kk] = b;
+                r = bf16_to_f32(b);
+            } else {
+                _Float16 hv = (_Float16)v;
+                o[kk] = __builtin_bit_cast(uint16_t, hv);
+                r = (float)hv;
+            }
+            s = __fadd_rn(s, __fmul_rn(r, r));
+        }
+    }
+    cn[i] = (i < N) ? (is_l2 ? s : 0.f) : INFINITY;
+}
+
+// ------------------------------------------------------------------------ top-k lists
+// Sorted ascending list; strict '<' keeps the earlier (smaller-label) entry first among equal
+// scores because every lane visits its candidates in ascending label order.
+template <int KL>
+__device__ __forceinline__ void list_insert(float (&ld)[KL], int (&li)[KL], float x, int xi) {
+#pragma unroll
+    for (int p = 0; p < KL; ++p) {
+        bool sw = x < ld[p];
+        float td = ld[p];
+        int ti = li[p];
+        ld[p] = sw ? x : td;
+        li[p] = sw ? xi : ti;
+        x = sw ? td : x;
+        xi = sw ? ti : xi;
+    }
+}
+
+template <int P>
+struct Frag;
+template <>
+struct Frag<PREC_FP32> {
+    using T = float;  // one k element per lane per 32x32x2 step
+    static constexpr int ESZ = 4;
+};
+template <>
+struct Frag<PREC_BF16> {
+    using T = uint4;  // eight bf16 per lane per 32x32x16 step
+    static constexpr int ESZ = 2;
+};
+template <>
+struct Frag<PREC_FP16> {
+    using T = uint4;
+    static constexpr int ESZ = 2;
+};
+
+template <int P>
+__device__ __forceinline__ f32x16_t mfma_step(typename Frag<P>::T a, typename Frag<P>::T b, f32x16_t c) {
+    if constexpr (P == PREC_FP32) {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+    } else if constexpr (P == PREC_BF16) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                       __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+    } else {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8_t, a),
+                                                      __builtin_bit_cast(f16x8_t, b), c, 0, 0, 0);
+    }
+}
+
+// MFMA 32x32 output layout: lane l holds column l%32 and rows 8*(i/4) + 4*(l/32) + (i%4),
+// i = 0..15. Candidates are the A rows, queries the B columns, so a lane holds one query.
+template <int P, int QB, int KL, int DK>
+__global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_, const float *__restrict__ cn,
+                                                   int64_t N, int64_t Npad, float beta,
+                                                   int32_t *__restrict__ cand) {
+    using FT = typename Frag<P>::T;
+    constexpr int ESZ = Frag<P>::ESZ;
+    constexpr int DP = 32 * DK;                       // padded row length (elements)
+    constexpr int KS = (P == PREC_FP32) ? DP / 2 : DP / 16;  // MFMA k-steps per row
+    constexpr int ROWB = DP * ESZ;                    // row bytes in HBM
+    constexpr int CB = (ROWB >= 512) ? 32 : 64;       // candidate rows per LDS stage
+    constexpr int LROWB = ROWB + 16;                  // padded LDS row (bank spread)
+    constexpr int NV = CB * ROWB / 16 / 256;          // uint4 per thread per stage
+    constexpr int CHUNKS = ROWB / 16;                 // uint4 per row
+    static_assert(NV >= 1 && NV * 256 * 16 == CB * ROWB, "stage must split evenly over 256 threads");
+
+    __shared__ __align__(16) unsigned char lds[2][CB * LROWB];
+
+    const unsigned char *Xh = (const unsigned char *)Xh_;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int h = lane >> 5;
+    const int col = lane & 31;
+    const int64_t q0 = ((int64_t)blockIdx.x * 4 + wave) * (32 * QB);
+
+    // query (B) fragments, resident for the whole candidate sweep
+    FT qf[QB][KS];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+        const unsigned char *row = Xh + (q0 + qb * 32 + col) * (int64_t)ROWB;
+        if constexpr (P == PREC_FP32) {
+#pragma unroll
+            for (int s = 0; s < KS; s += 4) {
+                float4 v = *(const float4 *)(row + (h * (DP / 2) + s) * 4);
+                qf[qb][s] = v.x;
+                qf[qb][s + 1] = v.y;
+                qf[qb][s + 2] = v.z;
+                qf[qb][s + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) qf[qb][s] = *(const uint4 *)(row + s * 32 + h * 16);
+        }
+    }
+
+    float ld[QB][KL];
+    int li[QB][KL];
+    float thr[QB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+#pragma unroll
+        for (int p = 0; p < KL; ++p) {
+            ld[qb][p] = INFINITY;
+            li[qb][p] = -1;
+        }
+        thr[qb] = INFINITY;
+    }
+
+    const int64_t nstages = Npad / CB;
+    uint4 stage[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(Xh + (int64_t)(v * 256 + threadIdx.x) * 16);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        int u = v * 256 + threadIdx.x;
+        int r = u / CHUNKS, c = u - r * CHUNKS;
+        *(uint4 *)(&lds[0][r * LROWB + c * 16]) = stage[v];
+    }
+    __syncthreads();
+
+    for (int64_t st = 0; st < nstages; ++st) {
+        const int buf = (int)(st & 1);
+        {
+            // next stage (the last iteration reloads its own stage; the copy is never read)
+            const int64_t nx = (st + 1 < nstages) ? st + 1 : st;
+            const unsigned char *nb = Xh + nx * CB * (int64_t)ROWB;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(nb + (int64_t)(v * 256 + threadIdx.x) * 16);
+        }
+#pragma unroll
+        for (int rb = 0; rb < CB / 32; ++rb) {
+            const int64_t c0 = st * CB + rb * 32;
+            const unsigned char *arow = &lds[buf][(rb * 32 + col) * LROWB];
+            f32x16_t acc[QB];
+#pragma unroll
+            for (int qb = 0; qb < QB; ++qb) acc[qb] = f32x16_t{};
+            if constexpr (P == PREC_FP32) {
+#pragma unroll
+                for (int s = 0; s < KS; s += 4) {
+                    float4 a4 = *(const float4 *)(arow + (h * (DP / 2) + s) * 4);
+#pragma unroll
+                    for (int qb = 0; qb < QB; ++qb) {
+                        acc[qb] = mfma_step<P>(a4.x, qf[qb][s], acc[qb]);
+                        acc[qb] = mfma_step<P>(a4.y, qf[qb][s + 1], acc[qb]);
+                        acc[qb] = mfma_step<P>(a4.z, qf[qb][s + 2], acc[qb]);
+                        acc[qb] = mfma_step<P>(a4.w, qf[qb][s + 3], acc[qb]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+                    uint4 a = *(const uint4 *)(arow + s * 32 + h * 16);
+#pragma unroll
+                    for (int qb = 0; qb < QB; ++qb) acc[qb] = mfma_step<P>(a, qf[qb][s], acc[qb]);
+                }
+            }
+            // score offsets of this lane's 16 candidate rows: c0 + 8g + 4h + j
+            float cnv[16];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                float4 v = *(const float4 *)(cn + c0 + 8 * g + 4 * h);
+                cnv[4 * g] = v.x;
+                cnv[4 * g + 1] = v.y;
+                cnv[4 * g + 2] = v.z;
+                cnv[4 * g + 3] = v.w;
+            }
+#pragma unroll
+            for (int qb = 0; qb < QB; ++qb) {
+                float m = INFINITY;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) m = fminf(m, fmaf(beta, acc[qb][i], cnv[i]));
+                if (__any(m < thr[qb])) {
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        float sc = fmaf(beta, acc[qb][i], cnv[i]);
+                        bool pass = sc < thr[qb];
+                        if (__any(pass)) {
+                            if (pass) {
+                                int ci = (int)(c0 + 8 * (i >> 2) + 4 * h + (i & 3));
+                                list_insert<KL>(ld[qb], li[qb], sc, ci);
+                                thr[qb] = ld[qb][KL - 1];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                int u = v * 256 + threadIdx.x;
+                int r = u / CHUNKS, c = u - r * CHUNKS;
+                *(uint4 *)(&lds[buf ^ 1][r * LROWB + c * 16]) = stage[v];
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+        int64_t q = q0 + qb * 32 + col;
+        if (q < N) {
+            int32_t *o = cand + q * (2 * KL) + h * KL;
+#pragma unroll
+            for (int p = 0; p < KL; ++p) o[p] = li[qb][p];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------ exact distances
+__device__ inline float exact_dist(const float *__restrict__ x, const float *__restrict__ y, int D, int is_l2) {
+    float s = 0.f;
+    if (is_l2) {
+        for (int j = 0; j < D; ++j) {
+            float t = __fsub_rn(x[j], y[j]);
+            s = __fadd_rn(s, __fmul_rn(t, t));
+        }
+        return s;
+    }
+    for (int j = 0; j < D; ++j) s = __fadd_rn(s, __fmul_rn(x[j], y[j]));
+    return __fsub_rn(1.0f, s);
+}
+
+__device__ inline bool key_less(float da, int ia, int la, float db, int ib, int lb) {
+    return da < db || (da == db && (ia < ib || (ia == ib && la < lb)));
+}
+
+// One wave per query; lane c reranks candidate c (C = 2*KL <= 64 candidates).
+__global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ Xp, int64_t N, int D, int is_l2,
+                                                    const int32_t *__restrict__ cand, int C, int k,
+                                                    int32_t *__restrict__ inds, float *__restrict__ dists,
+                                                    int *__restrict__ err) {
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= N) return;
+    int ci = (lane < C) ? cand[q * C + lane] : -1;
+    float dv = INFINITY;
+    if (ci >= 0) {
+        dv = exact_dist(Xp + q * D, Xp + (int64_t)ci * D, D, is_l2);
+        if (dv != dv) dv = INFINITY;  // NaN input: never ranks ahead of a real candidate
+    } else {
+        ci = INT_MAX;
+    }
+    int rank = 0;
+    for (int j = 0; j < 64; ++j) {
+        float dj = __shfl(dv, j);
+        int ij = __shfl(ci, j);
+        rank += key_less(dj, ij, j, dv, ci, lane) ? 1 : 0;
+    }
+    if (rank >= 1 && rank <= k) {
+        if (ci == INT_MAX) atomicExch(err, 1);
+        inds[q * k + (k - rank)] = (ci == INT_MAX) ? 0 : ci;
+        dists[q * k + (k - rank)] = dv;
+    }
+}
+
+// ------------------------------------------------------------------------ generic kNN
+// One thread per query: exact FP32 scan over all rows keeping a max-heap of the k+1 best
+// (distance, label) pairs in `heap` scratch ([N, k+1] pairs). All lanes of a wave visit the same
+// candidate row at the same time, so candidate reads are broadcasts.
+__global__ __launch_bounds__(256) void k_knn_generic(const float *__restrict__ Xp, int64_t N, int D, int is_l2,
+                                                     int k, float *__restrict__ hd_, int *__restrict__ hi_,
+                                                     int32_t *__restrict__ inds, float *__restrict__ dists) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N) return;
+    const int K1 = k + 1;
+    float *hd = hd_ + q * K1;
+    int *hi = hi_ + q * K1;
+    int cnt = 0;
+    const float *xq = Xp + q * D;
+    for (int64_t c = 0; c < N; ++c) {
+        float dv = exact_dist(xq, Xp + c * D, D, is_l2);
+        if (dv != dv) dv = INFINITY;
+        int ic = (int)c;
+        if (cnt < K1) {
+            // sift up
+            int pos = cnt++;
+            while (pos > 0) {
+                int par = (pos - 1) >> 1;
+                if (key_less(hd[par], hi[par], 0, dv, ic, 0)) {
+                    hd[pos] = hd[par];
+                    hi[pos] = hi[par];
+                    pos = par;
+                } else {
+                    break;
+                }
+            }
+            hd[pos] = dv;
+            hi[pos] = ic;
+        } else if (key_less(dv, ic, 0, hd[0], hi[0], 0)) {
+            // replace the max and sift down
+            int pos = 0;
+            while (true) {
+                int l = 2 * pos + 1, r = l + 1, big = pos;
+                float bd = dv;
+                int bi = ic;
+                if (l < K1 && key_less(bd, bi, 0, hd[l], hi[l], 0)) {
+                    big = l;
+                    bd = hd[l];
+                    bi = hi[l];
+                }
+                if (r < K1 && key_less(bd, bi, 0, hd[r], hi[r], 0)) big = r;
+                if (big == pos) break;
+                hd[pos] = hd[big];
+                hi[pos] = hi[big];
+                pos = big;
+            }
+            hd[pos] = dv;
+            hi[pos] = ic;
+        }
+    }
+    // heap-sort ascending in place, then drop the smallest and write farthest first
+    for (int end = cnt - 1; end > 0; --end) {
+        float td = hd[0];
+        int ti = hi[0];
+        float vd = hd[end];
+        int vi = hi[end];
+        hd[end] = td;
+        hi[end] = ti;
+        int pos = 0;
+        while (true) {
+            int l = 2 * pos + 1, r = l + 1, big = pos;
+            float bd = vd;
+            int bi = vi;
+            if (l < end && key_less(bd, bi, 0, hd[l], hi[l], 0)) {
+                big = l;
+                bd = hd[l];
+                bi = hi[l];
+            }
+            if (r < end && key_less(bd, bi, 0, hd[r], hi[r], 0)) big = r;
+            if (big == pos) break;
+            hd[pos] = hd[big];
+            hi[pos] = hi[big];
+            pos = big;
+        }
+        hd[pos] = vd;
+        hi[pos] = vi;
+    }
+    for (int j = 0; j < k; ++j) {
+        inds[q * k + j] = hi[K1 - 1 - j];
+        dists[q * k + j] = hd[K1 - 1 - j];
+    }
+}
+
+// ------------------------------------------------------------------------ host launcher
+template <int P, int QB, int KL>
+static void launch_tiles_dk(int DK, dim3 grid, const void *Xh, const float *cn, int64_t N, int64_t Npad,
+                            float beta, int32_t *cand, hipStream_t st) {
+    switch (DK) {
+        case 1: k_knn_tiles<P, QB, KL, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+        case 2: k_knn_tiles<P, QB, KL, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+        case 4: k_knn_tiles<P, QB, KL, 4><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+        case 8: k_knn_tiles<P, QB, KL, 8><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+        default: throw HipError("[ncvis_amd] unsupported padded dimension");
+    }
+}
+
+template <int P>
+static constexpr int qb_for() {
+    return P == PREC_FP32 ? 1 : 2;
+}
+
+template <int P>
+static void launch_tiles(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
+                         int32_t *cand, hipStream_t st) {
+    constexpr int QB = qb_for<P>();
+    dim3 grid((unsigned)(Npad / (4 * 32 * QB)));
+    if (KL == 16)
+        launch_tiles_dk<P, QB, 16>(DK, grid, Xh, cn, N, Npad, beta, cand, st);
+    else
+        launch_tiles_dk<P, QB, 32>(DK, grid, Xh, cn, N, Npad, beta, cand, st);
+}
+
+void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32_t *d_inds, float *d_dists,
+             hipStream_t st, KnnStats *stats) {
+    if (N < 2 || D < 1 || k < 1 || k > N - 1) throw HipError("[ncvis_amd] knn: invalid sizes");
+    if (N > (int64_t)INT_MAX - 512) throw HipError("[ncvis_amd] knn: more than 2^31 points");
+    const int is_l2 = (dist == DIST_L2);
+    DevBuf<float> xp;
+    const float *Xp = dX;
+    if (dist == DIST_COSINE || dist == DIST_CORRELATION) {
+        xp.alloc((size_t)N * D);
+        k_preprocess<<<cdiv(N, 256), 256, 0, st>>>(dX, N, D, dist, xp.p);
+        NCVIS_LAUNCH_CHECK();
+        Xp = xp.p;
+    }
+    DevBuf<int> err(1);
+    NCVIS_HIP_CHECK(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    hipEvent_t e0, e1, e2;
+    NCVIS_HIP_CHECK(hipEventCreate(&e0));
+    NCVIS_HIP_CHECK(hipEventCreate(&e1));
+    NCVIS_HIP_CHECK(hipEventCreate(&e2));
+    const bool tiled = (D <= 256) && (k + 1 <= 32);
+    if (tiled) {
+        const int DK = D <= 32 ? 1 : D <= 64 ? 2 : D <= 128 ? 4 : 8;
+        const int Dpad = 32 * DK;
+        const int QB = (prec == PREC_FP32) ? qb_for<PREC_FP32>() : qb_for<PREC_BF16>();
+        const int64_t qtile = 4 * 32 * QB;
+        const int64_t Npad = ((N + qtile - 1) / qtile) * qtile;
+        const int esz = (prec == PREC_FP32) ? 4 : 2;
+        DevBuf<unsigned char> xh((size_t)Npad * Dpad * esz);
+        DevBuf<float> cn((size_t)Npad);
+        switch (prec) {
+            case PREC_FP32:
+                k_pack<PREC_FP32><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, xh.p, cn.p);
+                break;
+            case PREC_BF16:
+                k_pack<PREC_BF16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, xh.p, cn.p);
+                break;
+            case PREC_FP16:
+                k_pack<PREC_FP16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, xh.p, cn.p);
+                break;
+            default: throw HipError("[ncvis_amd] knn: unknown precision");
+        }
+        NCVIS_LAUNCH_CHECK();
+        const int KL = (k + 1 <= 16) ? 16 : 32;
+        DevBuf<int32_t> cand((size_t)N * 2 * KL);
+        const float beta = is_l2 ? -2.0f : -1.0f;
+        NCVIS_HIP_CHECK(hipEventRecord(e0, st));
+        switch (prec) {
+            case PREC_FP32: launch_tiles<PREC_FP32>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
+            case PREC_BF16: launch_tiles<PREC_BF16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
+            default: launch_tiles<PREC_FP16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
+        }
+        NCVIS_LAUNCH_CHECK();
+        NCVIS_HIP_CHECK(hipEventRecord(e1, st));
+        k_knn_rerank<<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, cand.p, 2 * KL, k, d_inds, d_dists, err.p);
+        NCVIS_LAUNCH_CHECK();
+        NCVIS_HIP_CHECK(hipEventRecord(e2, st));
+    } else {
+        DevBuf<float> hd((size_t)N * (k + 1));
+        DevBuf<int> hi((size_t)N * (k + 1));
+        NCVIS_HIP_CHECK(hipEventRecord(e0, st));
+        k_knn_generic<<<cdiv(N, 256), 256, 0, st>>>(Xp, N, D, is_l2, k, hd.p, hi.p, d_inds, d_dists);
+        NCVIS_LAUNCH_CHECK();
+        NCVIS_HIP_CHECK(hipEventRecord(e1, st));
+        NCVIS_HIP_CHECK(hipEventRecord(e2, st));
+    }
+    int herr = 0;
+    NCVIS_HIP_CHECK(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    NCVIS_HIP_CHECK(hipStreamSynchronize(st));
+    if (stats) {
+        float t1 = 0, t2 = 0;
+        NCVIS_HIP_CHECK(hipEventElapsedTime(&t1, e0, e1));
+        NCVIS_HIP_CHECK(hipEventElapsedTime(&t2, e1, e2));
+        stats->tile_ms = t1;
+        stats->rerank_ms = t2;
+        stats->used_mfma = tiled ? 1 : 0;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (herr) throw HipError("[ncvis_amd] knn: fewer than k+1 finite candidates (non-finite input?)");
+}
+
+}  // namespace ncvis_hip

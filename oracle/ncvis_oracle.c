@@ -210,6 +210,42 @@ void orc_knn_bruteforce(const float *Xp, int64_t N, int64_t D, int64_t k, int di
     }
 }
 
+/* The same exact search for a subset of query rows q_idx[0..nq) (timing samples). */
+void orc_knn_bruteforce_queries(const float *Xp, int64_t N, int64_t D, int64_t k, int dist, int n_threads,
+                                const int64_t *q_idx, int64_t nq, int64_t *inds, float *dists) {
+    int64_t kk = k + 1;
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel
+    {
+        float *hd = (float *)malloc(sizeof(float) * kk);
+        int64_t *hi = (int64_t *)malloc(sizeof(int64_t) * kk);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t r = 0; r < nq; ++r) {
+            int64_t cnt = 0;
+            const float *xq = Xp + q_idx[r] * D;
+            for (int64_t c = 0; c < N; ++c) {
+                float dv = pair_dist(xq, Xp + c * D, D, dist);
+                if (cnt == kk && !pair_less(dv, c, hd[kk - 1], hi[kk - 1])) continue;
+                int64_t p = (cnt < kk) ? cnt++ : kk - 1;
+                while (p > 0 && pair_less(dv, c, hd[p - 1], hi[p - 1])) {
+                    hd[p] = hd[p - 1];
+                    hi[p] = hi[p - 1];
+                    --p;
+                }
+                hd[p] = dv;
+                hi[p] = c;
+            }
+            for (int64_t j = 0; j < k; ++j) {
+                int64_t src = cnt - 1 - j;
+                inds[r * k + j] = (src >= 1) ? hi[src] : -1;
+                dists[r * k + j] = (src >= 1) ? hd[src] : INFINITY;
+            }
+        }
+        free(hd);
+        free(hi);
+    }
+}
+
 /* -------------------------------------------- libstdc++ introsort over an index array */
 /* comp(a, b) := key[a] < key[b] (knntable.cpp:82-83) */
 #define KLESS(a, b) (key[(a)] < key[(b)])
@@ -636,10 +672,11 @@ double orc_nce_loss(int64_t N, int64_t d, int64_t E, const int64_t *src, const i
 /* ---------------------------------------------------------- fit_transform */
 /* ncvis.cpp:354-459 with exact brute-force kNN in place of HNSW. Stage wall times (seconds)
  * are written to times[6] = {buildKNN (0: brute force has no build), findKNN, symmetrize,
- * build_edges, initialize, optimize}. Returns 0 on success, -1 on bad input. */
+ * build_edges, initialize, optimize}; the final Q to *Q_out. Returns 0 on success, -1 on bad
+ * input. */
 int orc_fit_transform(const float *X, int64_t N, int64_t D, int64_t d, int n_threads, int64_t n_neighbors,
                       uint64_t seed, int n_epochs, int n_init_epochs, float a, float b, float alpha, float alpha_Q,
-                      const uint64_t *n_noise, int dist, float *Y, double *times) {
+                      const uint64_t *n_noise, int dist, float *Y, double *times, float *Q_out) {
     if (N == 0 || D == 0 || Y == NULL) return -1;
     double t0 = omp_get_wtime();
     float *Xp = (float *)malloc(sizeof(float) * (size_t)(N * D));
@@ -666,6 +703,7 @@ int orc_fit_transform(const float *X, int64_t N, int64_t D, int64_t d, int n_thr
     double t4 = omp_get_wtime();
     orc_optimize(N, d, E, src, col, Y, &Q, n_epochs, n_noise, a, b, alpha, alpha_Q, seed, n_threads);
     double t5 = omp_get_wtime();
+    if (Q_out) *Q_out = Q;
     if (times) {
         times[0] = 0.0;
         times[1] = t1 - t0;

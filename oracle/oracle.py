@@ -54,6 +54,8 @@ def lib():
         L.orc_preprocess.argtypes = [_f32, C.c_int64, C.c_int64, C.c_int, _f32]
         L.orc_knn_bruteforce.argtypes = [_f32, C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_int,
                                          _i64, _f32]
+        L.orc_knn_bruteforce_queries.argtypes = [_f32, C.c_int64, C.c_int64, C.c_int64, C.c_int,
+                                                 C.c_int, _i64, C.c_int64, _i64, _f32]
         L.orc_intro_sort_idx.argtypes = [_i64, C.c_int64, _f32]
         L.orc_symmetrize.argtypes = [_i64, _f32, C.c_int64, C.c_int64, _i64, _i64, _f32]
         L.orc_symmetrize.restype = C.c_int64
@@ -71,7 +73,8 @@ def lib():
         L.orc_nce_loss.restype = C.c_double
         L.orc_fit_transform.argtypes = [_f32, C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_int64,
                                         C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_float,
-                                        C.c_float, C.c_float, _u64, C.c_int, _f32, _f64]
+                                        C.c_float, C.c_float, _u64, C.c_int, _f32, _f64,
+                                        C.POINTER(C.c_float)]
         L.orc_fit_transform.restype = C.c_int
         _lib = L
     return _lib
@@ -133,6 +136,19 @@ def knn_bruteforce(Xp: np.ndarray, k: int, distance: str = "euclidean", n_thread
     inds = np.empty((N, k), np.int64)
     dists = np.empty((N, k), np.float32)
     lib().orc_knn_bruteforce(Xp, N, D, k, DIST_CODES[distance], n_threads, inds, dists)
+    return inds, dists
+
+
+def knn_bruteforce_queries(Xp: np.ndarray, k: int, queries, distance: str = "euclidean",
+                           n_threads: int = 0):
+    """knn_bruteforce restricted to the query rows ``queries`` (timing samples)."""
+    Xp = np.ascontiguousarray(Xp, np.float32)
+    N, D = Xp.shape
+    q = np.ascontiguousarray(queries, np.int64)
+    inds = np.empty((len(q), k), np.int64)
+    dists = np.empty((len(q), k), np.float32)
+    lib().orc_knn_bruteforce_queries(Xp, N, D, k, DIST_CODES[distance], n_threads, q, len(q), inds,
+                                     dists)
     return inds, dists
 
 
@@ -252,7 +268,7 @@ def fit_transform(X: np.ndarray, d: int = 2, n_threads: int = 1, n_neighbors: in
                   random_seed: int = 42, n_init_epochs: int = 20, a: float = 1.0, b: float = 1.0,
                   alpha: float = 1.0, alpha_Q: float = 1.0, n_noise=None, n_epochs: int = 50,
                   distance: str = "euclidean"):
-    """Full CPU pipeline (exact brute-force kNN in place of HNSW). Returns (Y, stage_times)."""
+    """Full CPU pipeline (exact brute-force kNN in place of HNSW). Returns (Y, stage_times, Q)."""
     X = np.ascontiguousarray(X, np.float32)
     N, D = X.shape
     if n_noise is None:
@@ -260,9 +276,10 @@ def fit_transform(X: np.ndarray, d: int = 2, n_threads: int = 1, n_neighbors: in
     nn = np.ascontiguousarray(n_noise, np.uint64)
     Y = np.empty((N, d), np.float32)
     times = np.zeros(6, np.float64)
+    q = C.c_float(0.0)
     rc = lib().orc_fit_transform(X, N, D, d, n_threads, n_neighbors, random_seed, len(nn),
                                  n_init_epochs, a, b, alpha, alpha_Q, nn, DIST_CODES[distance], Y,
-                                 times)
+                                 times, C.byref(q))
     if rc != 0:
         raise RuntimeError("oracle fit_transform rejected its input")
-    return Y, times
+    return Y, times, q.value

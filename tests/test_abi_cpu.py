@@ -1,0 +1,80 @@
+"""CPU-only checks of the boundary: the C-ABI library loads and exports every symbol the header
+declares; the Python layer validates arguments like the reference; no CPU fallback exists."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_symbols():
+    txt = open(os.path.join(ROOT, "include", "ncvis_amd.h")).read()
+    return sorted(set(re.findall(r"\b(ncvis_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from ncvis_amd import abi
+    lib = ctypes.CDLL(abi.LIB_PATH)
+    syms = _header_symbols()
+    assert len(syms) >= 13
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(abi.EXPORTS)
+
+
+def test_cython_extension_links_the_library():
+    import ncvis_amd
+    from ncvis_amd import _ncvis  # noqa: F401
+    assert callable(ncvis_amd.NCVis)
+
+
+def test_find_ab_params_closed_forms():
+    import ncvis_amd
+    a, b = ncvis_amd.find_ab_params(1.0, 0.4)
+    assert abs(a - 0.7668736385558027) < 1e-6 and abs(b - 1.2225638162909107) < 1e-6
+    a, b = ncvis_amd.find_ab_params(1.0, 0.1)
+    assert abs(a - 1.5769434602697652) < 1e-6 and abs(b - 0.8950608778515733) < 1e-6
+
+
+def _no_gpu():
+    try:
+        import torch
+        return not torch.cuda.is_available()
+    except Exception:
+        return True
+
+
+@pytest.mark.parametrize("kwargs,exc", [
+    (dict(n_noise=np.ones((2, 2), np.uint64)), ValueError),
+    (dict(n_noise=0), ValueError),
+    (dict(n_noise=2.5), ValueError),
+    (dict(distance="manhattan"), ValueError),
+    (dict(a=1.0), ValueError),
+    (dict(knn_precision="int8"), ValueError),
+])
+def test_python_layer_validation(kwargs, exc):
+    import ncvis_amd
+    with pytest.raises(exc):
+        ncvis_amd.NCVis(**kwargs)
+
+
+@pytest.mark.skipif(not _no_gpu(), reason="checks the no-GPU failure mode")
+def test_no_gpu_fails_loudly():
+    import ncvis_amd
+    with pytest.raises(RuntimeError):
+        ncvis_amd.NCVis()
+
+
+def test_default_negative_plan():
+    # linspace(0,1,50)**3 normalised to 250, rounded, clamped >= 1 (wrapper/ncvis.pyx:103-112)
+    want = [1] * 21 + [2] * 4 + [3] * 3 + [4] * 3 + [5] * 2 + [6] + [7] * 2 + [8] * 2 + [9, 10, 11, 11, 12, 13, 14, 15, 16, 17, 18, 20]
+    from ncvis_amd._ncvis import negative_plan_for
+    plan, ne = negative_plan_for(None, 50)
+    assert list(plan) == want and int(plan.sum()) == 262 and ne == 50
+    plan, ne = negative_plan_for(np.array([0, 2, 3]), 50)
+    assert list(plan) == [0, 2, 3] and ne == 3
+    plan, ne = negative_plan_for(4, 7)
+    assert list(plan) == [4] * 7 and ne == 7

@@ -1,0 +1,311 @@
+"""GPU parity of every stage of the fit path, through the C ABI, against the CPU oracle and the
+committed golden fixtures (made from the reference's own symmetrize)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from ncvis_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+A_DEF, B_DEF = 0.7668736385558027, 1.2225638162909107  # find_ab_params(1, 0.4)
+
+
+def _say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from ncvis_amd import abi
+    abi.lib()
+    return torch
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _stream(torch):
+    return torch.cuda.current_stream().cuda_stream
+
+
+def gpu_knn(torch, X, k, distance="euclidean", precision="fp32"):
+    from ncvis_amd import abi
+    N, D = X.shape
+    dX = _dev(torch, X.astype(np.float32))
+    inds = torch.empty((N, k), dtype=torch.int32, device="cuda")
+    dists = torch.empty((N, k), dtype=torch.float32, device="cuda")
+    abi.knn(dX.data_ptr(), N, D, k, distance, precision, inds.data_ptr(), dists.data_ptr(),
+            _stream(torch))
+    torch.cuda.synchronize()
+    return inds.cpu().numpy().astype(np.int64), dists.cpu().numpy()
+
+
+def gpu_symmetrize(torch, inds, dists):
+    from ncvis_amd import abi
+    N, k = inds.shape
+    di = _dev(torch, inds.astype(np.int32))
+    dd = _dev(torch, dists.astype(np.float32))
+    rp = torch.empty(N + 1, dtype=torch.int64, device="cuda")
+    col = torch.empty(2 * N * k, dtype=torch.int32, device="cuda")
+    dist = torch.empty(2 * N * k, dtype=torch.float32, device="cuda")
+    E = abi.symmetrize(di.data_ptr(), dd.data_ptr(), N, k, rp.data_ptr(), col.data_ptr(),
+                       dist.data_ptr(), _stream(torch))
+    torch.cuda.synchronize()
+    return rp.cpu().numpy(), col[:E].cpu().numpy().astype(np.int64), dist[:E].cpu().numpy()
+
+
+def gpu_init(torch, N, d, row_ptr, col, alpha, n_init, seed, T):
+    from ncvis_amd import abi
+    rp = _dev(torch, row_ptr.astype(np.int64))
+    cl = _dev(torch, col.astype(np.int32)) if len(col) else torch.zeros(1, dtype=torch.int32,
+                                                                          device="cuda")
+    Y = torch.empty((N, d), dtype=torch.float32, device="cuda")
+    abi.init_embedding(N, d, rp.data_ptr(), cl.data_ptr(), alpha, n_init, seed, T, Y.data_ptr(),
+                       _stream(torch))
+    torch.cuda.synchronize()
+    return Y.cpu().numpy()
+
+
+def gpu_optimize(torch, Y0, Q, row_ptr, col, plan, a, b, alpha, alpha_Q, seed, T, atomic=0):
+    from ncvis_amd import abi
+    N, d = Y0.shape
+    rp = _dev(torch, row_ptr.astype(np.int64))
+    cl = _dev(torch, col.astype(np.int32))
+    Y = _dev(torch, Y0.astype(np.float32))
+    q, trace = abi.optimize(N, d, rp.data_ptr(), cl.data_ptr(), len(col), Y.data_ptr(), Q, plan, a,
+                            b, alpha, alpha_Q, seed, T, atomic, _stream(torch))
+    torch.cuda.synchronize()
+    return Y.cpu().numpy(), q, trace
+
+
+# ------------------------------------------------------------------------------ RNG / init
+@pytest.mark.parametrize("T", [1, 3, 64])
+def test_init_random_layout_bit_exact(torch_cuda, T):
+    # n_init_epochs = 0 exposes the raw pcg64 / uniform_real<float> draws under T threads' static
+    # partition (ncvis.cpp:195-201)
+    _say(f"init random T={T}")
+    N, d = 1001, 2
+    rp = np.zeros(N + 1, np.int64)
+    col = np.zeros(0, np.int64)
+    got = gpu_init(torch_cuda, N, d, rp, col, 1.0, 0, 42, T)
+    want = oracle.init_embedding(N, d, rp, col, 1.0, 0, 42, T)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def _digits_graph(N=600, D=32, k=10, seed=3):
+    X, _ = synthetic.digits_like(N=N, D=D, seed=seed)
+    inds, dists = oracle.knn_bruteforce(oracle.preprocess(X), k)
+    row_ptr, col, _ = oracle.symmetrize(inds, dists)
+    return X, inds, dists, row_ptr, col
+
+
+@pytest.mark.parametrize("n_init", [1, 4, 7])
+def test_init_embedding_matches_oracle(torch_cuda, n_init):
+    _say(f"init n_init={n_init}")
+    X, inds, dists, row_ptr, col = _digits_graph()
+    N, k = inds.shape
+    got = gpu_init(torch_cuda, N, 2, row_ptr, col, np.float32(1.0 / k), n_init, 42, 1)
+    want = oracle.init_embedding(N, 2, row_ptr, col, np.float32(1.0 / k), n_init, 42, 1)
+    np.testing.assert_allclose(got, want, atol=1e-5)
+
+
+# ------------------------------------------------------------------------------ kNN
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("shape", [(1500, 64, 15), (700, 32, 5), (333, 100, 31), (40, 16, 15)])
+def test_knn_exact_on_integer_data(torch_cuda, precision, shape):
+    # integer data: every precision computes exact candidate scores, so the table equals the
+    # exact (distance, label) order bit for bit, ties included
+    N, D, k = shape
+    _say(f"knn int N={N} D={D} k={k} {precision}")
+    X, _ = synthetic.digits_like(N=N, D=D, seed=N + D)
+    gi, gd = gpu_knn(torch_cuda, X, k, "euclidean", precision)
+    oi, od = oracle.knn_bruteforce(oracle.preprocess(X), k)
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(gd, od)
+
+
+@pytest.mark.parametrize("shape", [(300, 300, 15), (200, 20, 40), (5, 3, 4)])
+def test_knn_generic_path(torch_cuda, shape):
+    N, D, k = shape
+    _say(f"knn generic N={N} D={D} k={k}")
+    X, _ = synthetic.digits_like(N=N, D=D, seed=7)
+    gi, gd = gpu_knn(torch_cuda, X, k, "euclidean", "fp32")
+    oi, od = oracle.knn_bruteforce(oracle.preprocess(X), k)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+
+
+@pytest.mark.parametrize("distance", ["cosine", "correlation", "inner_product"])
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_knn_inner_product_family(torch_cuda, distance, precision):
+    _say(f"knn {distance} {precision}")
+    X, _ = synthetic.gaussian_mixture(800, 24, 6, seed=5)
+    gi, gd = gpu_knn(torch_cuda, X, 15, distance, precision)
+    oi, od = oracle.knn_bruteforce(oracle.preprocess(X, distance), 15, distance)
+    # exact FP32 rerank: distances agree to rounding; identical neighbour sets almost everywhere
+    agree = np.mean([len(set(a) & set(b)) / 15 for a, b in zip(gi, oi)])
+    assert agree > 0.995
+    np.testing.assert_allclose(np.sort(gd, 1), np.sort(od, 1), atol=2e-5, rtol=1e-5)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+def test_knn_recall_float_data(torch_cuda, precision):
+    _say(f"knn recall {precision}")
+    X, _ = synthetic.anisotropic_mixture(N=4000, D=128, n_clusters=10, seed=1)
+    gi, _ = gpu_knn(torch_cuda, X, 15, "euclidean", precision)
+    q = np.arange(0, 4000, 7)
+    oi, _ = oracle.knn_fp64(oracle.preprocess(X), 15, "euclidean", queries=q)
+    recall = np.mean([len(set(a) & set(b)) / 15 for a, b in zip(gi[q], oi)])
+    assert recall >= 0.99, recall
+
+
+# ------------------------------------------------------------------------------ symmetrize
+@pytest.mark.parametrize("name", ["sym_digits1500.npz", "sym_ties_hubs.npz", "sym_unsorted.npz",
+                                  "sym_tiny.npz"])
+def test_symmetrize_bit_exact_vs_reference_fixture(torch_cuda, golden_dir, name):
+    _say(f"symmetrize fixture {name}")
+    z = np.load(os.path.join(golden_dir, name))
+    rp, col, dist = gpu_symmetrize(torch_cuda, z["inds"].astype(np.int64), z["dists"])
+    assert np.array_equal(rp, z["row_ptr"])
+    assert np.array_equal(col, z["col"].astype(np.int64))
+    assert np.array_equal(dist.view(np.uint32), z["dist"].view(np.uint32))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_symmetrize_bit_exact_random_tables(torch_cuda, seed):
+    _say(f"symmetrize random seed={seed}")
+    rng = np.random.default_rng(seed)
+    N, k = int(rng.integers(50, 3000)), int(rng.integers(1, 24))
+    levels = int(rng.choice([2, 3, 50, 100000]))
+    inds = np.stack([rng.choice(np.delete(np.arange(N), i), k, replace=False) for i in range(N)])
+    hubs = rng.choice(N, 5, replace=False)
+    mask = rng.random((N, k)) < 0.2
+    for i, j in zip(*np.nonzero(mask)):
+        h = hubs[j % 5]
+        if h != i and h not in inds[i]:
+            inds[i, j] = h
+    dists = (rng.integers(1, levels + 1, (N, k)) / levels).astype(np.float32)
+    got = gpu_symmetrize(torch_cuda, inds, dists)
+    want = oracle.symmetrize(inds, dists)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+def test_knn_then_symmetrize_digits(torch_cuda):
+    _say("knn+symmetrize digits 5620x64")
+    X, _ = synthetic.digits_like(seed=0)
+    gi, gd = gpu_knn(torch_cuda, X, 15, "euclidean", "fp16")
+    got = gpu_symmetrize(torch_cuda, gi, gd)
+    oi, od = oracle.knn_bruteforce(oracle.preprocess(X), 15)
+    want = oracle.symmetrize(oi, od)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+# ------------------------------------------------------------------------------ SGD epochs
+@pytest.mark.parametrize("plan", [[1], [3], [0], [5, 2]])
+def test_optimize_single_chain_matches_reference(torch_cuda, plan):
+    # n_threads = 1, fixed seed, sequential edge order: one chain == the reference at one thread
+    _say(f"optimize T=1 plan={plan}")
+    X, inds, dists, row_ptr, col = _digits_graph()
+    N, k = inds.shape
+    Y0 = oracle.init_embedding(N, 2, row_ptr, col, np.float32(1.0 / k), 5, 42, 1)
+    gy, gq, _ = gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 1.0, 1.0, 42, 1)
+    oy, oq = oracle.optimize(Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 1.0, 1.0, 42, 1)
+    np.testing.assert_allclose(gy, oy, atol=1e-4, rtol=0)
+    assert abs(gq - oq) < 1e-4 * max(1.0, abs(oq))
+
+
+def test_optimize_single_chain_d1_and_d3(torch_cuda):
+    X, inds, dists, row_ptr, col = _digits_graph(N=300)
+    N, k = inds.shape
+    for d in (1, 3, 5):
+        _say(f"optimize T=1 d={d}")
+        Y0 = oracle.init_embedding(N, d, row_ptr, col, np.float32(1.0 / k), 3, 42, 1)
+        gy, gq, _ = gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, [2], A_DEF, B_DEF, 1.0, 1.0, 7, 1)
+        oy, oq = oracle.optimize(Y0, 0.0, row_ptr, col, [2], A_DEF, B_DEF, 1.0, 1.0, 7, 1)
+        np.testing.assert_allclose(gy, oy, atol=1e-4, rtol=0)
+
+
+def test_optimize_chains_statistics(torch_cuda):
+    # many chains (Hogwild): not bitwise comparable, but Q must settle like the reference's
+    _say("optimize chains statistics")
+    X, inds, dists, row_ptr, col = _digits_graph(N=2000, D=32, k=15)
+    N, k = inds.shape
+    Y0 = oracle.init_embedding(N, 2, row_ptr, col, np.float32(1.0 / k), 20, 42, 1)
+    plan = [3] * 10
+    for atomic in (0, 1):
+        gy, gq, trace = gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 1.0, 1.0,
+                                     42, 64, atomic)
+        oy, oq = oracle.optimize(Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 1.0, 1.0, 42, 4)
+        assert np.all(np.isfinite(gy))
+        assert abs(gq - oq) < 0.5 + 0.1 * abs(oq), (gq, oq)
+
+
+# ------------------------------------------------------------------------------ NCE loss
+@pytest.mark.parametrize("d", [2, 16])
+def test_nce_loss_and_adjoint_vs_oracle(torch_cuda, d):
+    from ncvis_amd import abi
+    torch = torch_cuda
+    _say(f"nce loss d={d}")
+    X, inds, dists, row_ptr, col = _digits_graph(N=400)
+    N = inds.shape[0]
+    rng = np.random.default_rng(d)
+    Y = rng.normal(size=(N, d)).astype(np.float32) * 3
+    E, n = len(col), 5
+    neg = rng.integers(0, N, (E, n))
+    Q = -2.5
+    src, dst = oracle.edges_from_csr(row_ptr, col)
+    L, dY, dQ = oracle.nce_loss(Y, Q, src, dst, neg, A_DEF, B_DEF)
+    rp = _dev(torch, row_ptr)
+    cl = _dev(torch, col.astype(np.int32))
+    ng = _dev(torch, neg.astype(np.int32))
+    Yd = _dev(torch, Y)
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    gQ = torch.zeros(1, dtype=torch.float64, device="cuda")
+    gY = torch.zeros((N, d), dtype=torch.float32, device="cuda")
+    abi.nce_loss(N, d, rp.data_ptr(), cl.data_ptr(), E, ng.data_ptr(), n, Yd.data_ptr(), Q, A_DEF,
+                 B_DEF, loss.data_ptr(), gY.data_ptr(), gQ.data_ptr(), _stream(torch))
+    torch.cuda.synchronize()
+    assert abs(loss.item() - L) < 1e-5 * abs(L)
+    assert abs(gQ.item() - dQ) < 1e-4 * max(1, abs(dQ))
+    np.testing.assert_allclose(gY.cpu().numpy(), dY, rtol=1e-3, atol=1e-4 * np.abs(dY).max())
+
+
+# ------------------------------------------------------------------------------ full pipeline
+def test_fit_transform_single_chain_matches_reference(torch_cuda):
+    import ncvis_amd
+    _say("fit_transform T=1 vs oracle")
+    X, _ = synthetic.digits_like(N=800, D=32, seed=9)
+    plan = np.array([1, 2], np.uint64)
+    vis = ncvis_amd.NCVis(n_threads=1, n_neighbors=15, n_noise=plan, n_init_epochs=20,
+                          knn_precision="fp32", random_seed=42)
+    Y = vis.fit_transform(X)
+    Yo, _, _ = oracle.fit_transform(X, d=2, n_threads=1, n_neighbors=15, random_seed=42,
+                                 n_init_epochs=20, a=vis.a, b=vis.b, n_noise=plan)
+    np.testing.assert_allclose(Y, Yo, atol=1e-4, rtol=0)
+
+
+def test_fit_transform_handle_device_api(torch_cuda):
+    from ncvis_amd import abi
+    torch = torch_cuda
+    _say("handle device api")
+    X, _ = synthetic.digits_like(N=500, D=16, seed=2)
+    h = abi.Handle(d=2, n_threads=0, n_neighbors=10, n_noise=[1, 2, 3], a=A_DEF, b=B_DEF)
+    dX = _dev(torch, X)
+    dY = torch.empty((500, 2), dtype=torch.float32, device="cuda")
+    h.fit_transform_device(dX.data_ptr(), 500, 16, dY.data_ptr(), _stream(torch))
+    torch.cuda.synchronize()
+    s = h.stats()
+    assert s["n_edges"] > 500 * 10 and s["chains"] >= 1
+    assert np.all(np.isfinite(dY.cpu().numpy()))
+    with pytest.raises(RuntimeError, match="at least one element"):
+        h.fit_transform_device(dX.data_ptr(), 0, 16, dY.data_ptr())
