@@ -13,6 +13,8 @@
 //   k_sgd_epoch    T chains, chain t = the reference's OpenMP thread t: pcg64(seed + t), the
 //                  static edge chunk [s0, e0) of thread t, a private Q copy, sequential samples.
 //                  Chains run concurrently Hogwild-style; T = 1 is the reference at one thread.
+//   k_sgd_epoch_hw the many-chain form: source row kept in registers with the chain's own
+//                  updates, other endpoints updated with atomics, negatives' rows prefetched.
 //   k_q_reduce     Q = mean of the chains' Q copies (ncvis.cpp:342-349), FP64 sum.
 // NCE loss (the objective the update of ncvis.cpp:304-338 descends):
 //   k_nce_loss     per edge: positive + n explicit negatives; loss and dL/dQ as FP64 partial
@@ -137,6 +139,11 @@ void init_embedding_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_
 }
 
 // ------------------------------------------------------------------------ SGD
+// powf / expf evaluated in double and rounded once: correctly rounded in all but vanishingly
+// rare cases, which is what the host libm's float versions return almost always.
+__device__ __forceinline__ float pow_rn(float x, float y) { return (float)pow((double)x, (double)y); }
+__device__ __forceinline__ float exp_rn(float x) { return (float)exp((double)x); }
+
 struct SampleConst {
     float a, b, alpha_Q, step;
     float noise_f;     // (float)cur_noise
@@ -154,16 +161,16 @@ __device__ __forceinline__ void sgd_math(float (&yi)[DIM], float (&yo)[DIM], boo
         float t = __fsub_rn(yi[k], yo[k]);
         d2 = __fadd_rn(d2, __fmul_rn(t, t));
     }
-    float Ph = 1.f / (1.f + c.a * powf(d2, c.b));
+    float Ph = 1.f / __fadd_rn(1.f, __fmul_rn(c.a, pow_rn(d2, c.b)));
     float w = 1.f;
     if (c.noise != 0) {
-        w = Ph / (c.noise_f * expf(Qc));
+        w = Ph / (c.noise_f * exp_rn(Qc));
         if (positive)
             w = 1.f / (1.f + w);
         else
             w = -1.f / (1.f + 1.f / w);
         Qc = __fsub_rn(Qc, __fmul_rn(w, c.alpha_Q));
-        w = 2.f * w * Ph * c.a * c.b * powf(d2, c.b - 1.f);
+        w = 2.f * w * Ph * c.a * c.b * pow_rn(d2, c.b - 1.f);
     }
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
@@ -212,16 +219,16 @@ __device__ inline void sgd_sample_rt(float *__restrict__ Y, int d, int64_t id, i
         float t = __fsub_rn(Y[id * d + k], Y[other * d + k]);
         d2 = __fadd_rn(d2, __fmul_rn(t, t));
     }
-    float Ph = 1.f / (1.f + c.a * powf(d2, c.b));
+    float Ph = 1.f / __fadd_rn(1.f, __fmul_rn(c.a, pow_rn(d2, c.b)));
     float w = 1.f;
     if (c.noise != 0) {
-        w = Ph / (c.noise_f * expf(Qc));
+        w = Ph / (c.noise_f * exp_rn(Qc));
         if (positive)
             w = 1.f / (1.f + w);
         else
             w = -1.f / (1.f + 1.f / w);
         Qc = __fsub_rn(Qc, __fmul_rn(w, c.alpha_Q));
-        w = 2.f * w * Ph * c.a * c.b * powf(d2, c.b - 1.f);
+        w = 2.f * w * Ph * c.a * c.b * pow_rn(d2, c.b - 1.f);
     }
     for (int k = 0; k < d; ++k) {
         float v = __fmul_rn(__fmul_rn(__fsub_rn(Y[other * d + k], Y[id * d + k]), w), c.step);
@@ -298,6 +305,94 @@ __global__ __launch_bounds__(256) void k_sgd_epoch(float *__restrict__ Y, int d_
     qpart[t] = Qc;
 }
 
+// Many-chain (Hogwild) epoch. A chain keeps the row of its current source point in registers
+// and adds its own updates of that row there (the chain sees its own writes, as a thread of the
+// reference does), flushing them to Y with atomics when it moves to the next row; updates of
+// the other endpoint go to Y with atomics at once, so concurrent chains lose no update. The
+// negatives of an edge are drawn before their rows are read (the draw sequence does not depend
+// on Y), so PF rows are fetched together instead of one dependent load per sample.
+template <int DIM, int PF>
+__global__ __launch_bounds__(256) void k_sgd_epoch_hw(float *__restrict__ Y, const int64_t *__restrict__ row_ptr,
+                                                      const int32_t *__restrict__ col, int64_t E, int64_t T,
+                                                      const int64_t *__restrict__ chain_row, U128 *__restrict__ rng,
+                                                      const float *__restrict__ Qdev, float *__restrict__ qpart,
+                                                      SampleConst c, uint64_t N) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    int64_t s0, e0;
+    static_chunk(E, T, t, s0, e0);
+    float Qc = *Qdev;
+    U128 g = rng[t];
+    int64_t r = chain_row[t];
+    int64_t rend = row_ptr[r + 1];
+    float base[DIM], acc[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+        base[k] = Y[r * DIM + k];
+        acc[k] = 0.f;
+    }
+    const uint32_t total = c.noise + 1;
+    for (int64_t e = s0; e < e0; ++e) {
+        while (e >= rend) {
+#pragma unroll
+            for (int k = 0; k < DIM; ++k)
+                if (acc[k] != 0.f) __hip_atomic_fetch_add(&Y[r * DIM + k], acc[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ++r;
+            rend = row_ptr[r + 1];
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) {
+                base[k] = Y[r * DIM + k];
+                acc[k] = 0.f;
+            }
+        }
+        const int64_t id = r;
+        for (uint32_t j0 = 0; j0 < total; j0 += PF) {
+            int64_t oth[PF];
+            float yo[PF][DIM];
+#pragma unroll
+            for (int p = 0; p < PF; ++p) {
+                const uint32_t j = j0 + p;
+                oth[p] = id;
+                if (j < total) {
+                    if (j == 0) {
+                        oth[p] = col[e];
+                    } else {
+                        int64_t o;
+                        do {
+                            o = (int64_t)uniform_index(g, N);
+                        } while (o == id);
+                        oth[p] = o;
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < PF; ++p) {
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) yo[p][k] = Y[oth[p] * DIM + k];
+            }
+#pragma unroll
+            for (int p = 0; p < PF; ++p) {
+                if (j0 + p < total) {
+                    float yi[DIM], dx[DIM];
+#pragma unroll
+                    for (int k = 0; k < DIM; ++k) yi[k] = __fadd_rn(base[k], acc[k]);
+                    sgd_math<DIM>(yi, yo[p], (j0 + p) == 0, c, Qc, dx);
+#pragma unroll
+                    for (int k = 0; k < DIM; ++k) {
+                        acc[k] = __fadd_rn(acc[k], dx[k]);
+                        __hip_atomic_fetch_add(&Y[oth[p] * DIM + k], -dx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < DIM; ++k)
+        if (acc[k] != 0.f) __hip_atomic_fetch_add(&Y[r * DIM + k], acc[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    rng[t] = g;
+    qpart[t] = Qc;
+}
+
 __global__ __launch_bounds__(256) void k_q_reduce(const float *__restrict__ qpart, int64_t T, float *__restrict__ Qdev,
                                                   float *__restrict__ trace, int epoch) {
     __shared__ double red[256];
@@ -319,6 +414,16 @@ __global__ __launch_bounds__(256) void k_q_reduce(const float *__restrict__ qpar
 static void launch_epoch(int d, dim3 grid, float *Y, const int64_t *row_ptr, const int32_t *col, int64_t E,
                          int64_t T, const int64_t *chain_row, U128 *rng, const float *Qdev, float *qpart,
                          const SampleConst &c, uint64_t N, int atomic_mode, hipStream_t st) {
+    constexpr int PF = 4;
+    if (atomic_mode && d >= 1 && d <= 4) {
+        switch (d) {
+            case 1: k_sgd_epoch_hw<1, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N); break;
+            case 2: k_sgd_epoch_hw<2, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N); break;
+            case 3: k_sgd_epoch_hw<3, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N); break;
+            default: k_sgd_epoch_hw<4, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N); break;
+        }
+        return;
+    }
     switch (d) {
         case 1: k_sgd_epoch<1><<<grid, 256, 0, st>>>(Y, d, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, atomic_mode); break;
         case 2: k_sgd_epoch<2><<<grid, 256, 0, st>>>(Y, d, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, atomic_mode); break;
