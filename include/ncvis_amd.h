@@ -60,6 +60,11 @@ const char *ncvis_last_error(const ncvis_handle *h);
  *   "chains"          override the SGD chain count (0 = from n_threads) */
 int ncvis_set_option(ncvis_handle *h, const char *key, double value);
 
+/* Process-wide tuning knobs (no reference counterpart; used by the autotuner). None changes a
+ * single-chain fit. Keys: "sgd_prefetch" (1,2,4,8), "sgd_plain_far" (0/1), "sgd_fast_math"
+ * (0/1), "edges_per_chain" (auto chain count = E / this), "max_chains". */
+int ncvis_tune(const char *key, double value);
+
 /* ncvis::NCVis::fit_transform: X [N, D] and Y [N, d] are host, row-major float32. */
 int ncvis_fit_transform(ncvis_handle *h, const float *X, size_t N, size_t D, float *Y);
 

@@ -16,7 +16,7 @@ EXPORTS = (
     "ncvis_create", "ncvis_destroy", "ncvis_last_error", "ncvis_set_option", "ncvis_fit_transform",
     "ncvis_fit_transform_device", "ncvis_get_stats", "ncvis_get_last_error", "ncvis_knn_device",
     "ncvis_symmetrize_device", "ncvis_init_embedding_device", "ncvis_optimize_device",
-    "ncvis_nce_loss_device",
+    "ncvis_nce_loss_device", "ncvis_tune",
 )
 
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
@@ -55,6 +55,7 @@ def lib():
                                             C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_size_t),
                                             C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint64,
                                             C.c_int64, C.c_int, vp, vp]
+        L.ncvis_tune.argtypes = [C.c_char_p, C.c_double]
         L.ncvis_nce_loss_device.argtypes = [C.c_size_t, C.c_size_t, vp, vp, C.c_int64, vp, C.c_int,
                                             vp, C.c_float, C.c_float, C.c_float, vp, vp, vp, vp]
         for name in EXPORTS:
@@ -72,6 +73,11 @@ def _check(rc: int) -> None:
 
 def _stream(stream) -> int:
     return 0 if stream is None else int(stream)
+
+
+def tune(key: str, value: float) -> None:
+    """Set a process-wide tuning knob (ncvis_tune)."""
+    _check(lib().ncvis_tune(key.encode(), float(value)))
 
 
 def knn(dX: int, N: int, D: int, k: int, distance: str, precision: str, d_inds: int, d_dists: int,

@@ -32,6 +32,13 @@ struct ncvis_handle {
 
 static thread_local std::string g_last_error;
 
+namespace ncvis_hip {
+Tuning &tuning() {
+    static Tuning t;
+    return t;
+}
+}  // namespace ncvis_hip
+
 template <typename F>
 static int guarded(std::string &err, F &&f) {
     try {
@@ -53,8 +60,11 @@ static int guarded(std::string &err, F &&f) {
     }
 }
 
-// GPU-sized number of SGD chains: about 64 edges per chain, at most 2^18 chains.
-static int64_t auto_chains(int64_t E) { return std::max<int64_t>(1, std::min<int64_t>(E / 64, int64_t(1) << 18)); }
+// GPU-sized number of SGD chains: E / edges_per_chain, at most max_chains (tuning knobs).
+static int64_t auto_chains(int64_t E) {
+    const Tuning &t = tuning();
+    return std::max<int64_t>(1, std::min<int64_t>(E / std::max<int64_t>(1, t.edges_per_chain), t.max_chains));
+}
 
 static int64_t resolve_chains(const ncvis_handle *h, int64_t E) {
     if (h->chains_override > 0) return h->chains_override;
@@ -129,6 +139,37 @@ int ncvis_set_option(ncvis_handle *h, const char *key, double value) {
             h->chains_override = (int64_t)value;
         } else {
             throw std::invalid_argument("[ncvis_amd] unknown option '" + k + "'");
+        }
+    });
+}
+
+int ncvis_tune(const char *key, double value) {
+    return guarded(g_last_error, [&] {
+        std::string k = key ? key : "";
+        Tuning &t = tuning();
+        if (k == "sgd_prefetch") {
+            int v = (int)value;
+            if (v != 1 && v != 2 && v != 4 && v != 8) throw std::invalid_argument("[ncvis_amd] sgd_prefetch: 1, 2, 4 or 8");
+            t.sgd_prefetch = v;
+        } else if (k == "sgd_plain_far") {
+            t.sgd_plain_far = value != 0;
+        } else if (k == "sgd_fast_math") {
+            t.sgd_fast_math = value != 0;
+        } else if (k == "edges_per_chain") {
+            if (value < 1) throw std::invalid_argument("[ncvis_amd] edges_per_chain must be >= 1");
+            t.edges_per_chain = (int64_t)value;
+        } else if (k == "knn_dma") {
+            t.knn_dma = value < 0 ? -1 : (value != 0);
+        } else if (k == "knn_stage_mult") {
+            if (value != 1 && value != 2) throw std::invalid_argument("[ncvis_amd] knn_stage_mult: 1 or 2");
+            t.knn_stage_mult = (int)value;
+        } else if (k == "knn_select") {
+            t.knn_select = value < 0 ? -1 : (value != 0);
+        } else if (k == "max_chains") {
+            if (value < 1) throw std::invalid_argument("[ncvis_amd] max_chains must be >= 1");
+            t.max_chains = (int64_t)value;
+        } else {
+            throw std::invalid_argument("[ncvis_amd] unknown tuning key '" + k + "'");
         }
     });
 }

@@ -14,7 +14,8 @@
 //                  static edge chunk [s0, e0) of thread t, a private Q copy, sequential samples.
 //                  Chains run concurrently Hogwild-style; T = 1 is the reference at one thread.
 //   k_sgd_epoch_hw the many-chain form: source row kept in registers with the chain's own
-//                  updates, other endpoints updated with atomics, negatives' rows prefetched.
+//                  updates, other endpoints updated atomically (one 64-bit compare-and-swap per
+//                  coordinate pair), negatives' rows prefetched.
 //   k_q_reduce     Q = mean of the chains' Q copies (ncvis.cpp:342-349), FP64 sum.
 // NCE loss (the objective the update of ncvis.cpp:304-338 descends):
 //   k_nce_loss     per edge: positive + n explicit negatives; loss and dL/dQ as FP64 partial
@@ -148,6 +149,7 @@ struct SampleConst {
     float a, b, alpha_Q, step;
     float noise_f;     // (float)cur_noise
     uint32_t noise;    // cur_noise
+    int fast_math;     // 1: single-precision powf / expf
 };
 
 // The sample body of ncvis.cpp:304-339 on a register copy of the two rows. Returns the weight
@@ -161,16 +163,16 @@ __device__ __forceinline__ void sgd_math(float (&yi)[DIM], float (&yo)[DIM], boo
         float t = __fsub_rn(yi[k], yo[k]);
         d2 = __fadd_rn(d2, __fmul_rn(t, t));
     }
-    float Ph = 1.f / __fadd_rn(1.f, __fmul_rn(c.a, pow_rn(d2, c.b)));
+    float Ph = 1.f / __fadd_rn(1.f, __fmul_rn(c.a, c.fast_math ? powf(d2, c.b) : pow_rn(d2, c.b)));
     float w = 1.f;
     if (c.noise != 0) {
-        w = Ph / (c.noise_f * exp_rn(Qc));
+        w = Ph / (c.noise_f * (c.fast_math ? expf(Qc) : exp_rn(Qc)));
         if (positive)
             w = 1.f / (1.f + w);
         else
             w = -1.f / (1.f + 1.f / w);
         Qc = __fsub_rn(Qc, __fmul_rn(w, c.alpha_Q));
-        w = 2.f * w * Ph * c.a * c.b * pow_rn(d2, c.b - 1.f);
+        w = 2.f * w * Ph * c.a * c.b * (c.fast_math ? powf(d2, c.b - 1.f) : pow_rn(d2, c.b - 1.f));
     }
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
@@ -316,7 +318,7 @@ __global__ __launch_bounds__(256) void k_sgd_epoch_hw(float *__restrict__ Y, con
                                                       const int32_t *__restrict__ col, int64_t E, int64_t T,
                                                       const int64_t *__restrict__ chain_row, U128 *__restrict__ rng,
                                                       const float *__restrict__ Qdev, float *__restrict__ qpart,
-                                                      SampleConst c, uint64_t N) {
+                                                      SampleConst c, uint64_t N, int plain_far) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
     int64_t s0, e0;
@@ -378,9 +380,33 @@ __global__ __launch_bounds__(256) void k_sgd_epoch_hw(float *__restrict__ Y, con
                     for (int k = 0; k < DIM; ++k) yi[k] = __fadd_rn(base[k], acc[k]);
                     sgd_math<DIM>(yi, yo[p], (j0 + p) == 0, c, Qc, dx);
 #pragma unroll
-                    for (int k = 0; k < DIM; ++k) {
-                        acc[k] = __fadd_rn(acc[k], dx[k]);
-                        __hip_atomic_fetch_add(&Y[oth[p] * DIM + k], -dx[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    for (int k = 0; k < DIM; ++k) acc[k] = __fadd_rn(acc[k], dx[k]);
+                    if (plain_far) {
+#pragma unroll
+                        for (int k = 0; k < DIM; ++k) Y[oth[p] * DIM + k] = __fsub_rn(yo[p][k], dx[k]);
+                    } else if constexpr (DIM % 2 == 0) {
+                        // one 64-bit compare-and-swap per coordinate pair: the same float
+                        // subtraction as two float atomics, at half the atomic operations; the
+                        // prefetched row is the expected value, a concurrent update retries
+#pragma unroll
+                        for (int k = 0; k < DIM; k += 2) {
+                            unsigned long long *addr = (unsigned long long *)&Y[oth[p] * DIM + k];
+                            unsigned long long expect =
+                                __builtin_bit_cast(unsigned long long, make_float2(yo[p][k], yo[p][k + 1]));
+                            while (true) {
+                                float2 v = __builtin_bit_cast(float2, expect);
+                                float2 nv = make_float2(__fsub_rn(v.x, dx[k]), __fsub_rn(v.y, dx[k + 1]));
+                                unsigned long long prev =
+                                    atomicCAS(addr, expect, __builtin_bit_cast(unsigned long long, nv));
+                                if (prev == expect) break;
+                                expect = prev;
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < DIM; ++k)
+                            __hip_atomic_fetch_add(&Y[oth[p] * DIM + k], -dx[k], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
                     }
                 }
             }
@@ -411,16 +437,30 @@ __global__ __launch_bounds__(256) void k_q_reduce(const float *__restrict__ qpar
     }
 }
 
+template <int PF>
+static void launch_hw(int d, dim3 grid, float *Y, const int64_t *row_ptr, const int32_t *col, int64_t E, int64_t T,
+                      const int64_t *chain_row, U128 *rng, const float *Qdev, float *qpart, const SampleConst &c,
+                      uint64_t N, int plain_far, hipStream_t st) {
+    switch (d) {
+        case 1: k_sgd_epoch_hw<1, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, plain_far); break;
+        case 2: k_sgd_epoch_hw<2, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, plain_far); break;
+        case 3: k_sgd_epoch_hw<3, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, plain_far); break;
+        default: k_sgd_epoch_hw<4, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, plain_far); break;
+    }
+}
+
 static void launch_epoch(int d, dim3 grid, float *Y, const int64_t *row_ptr, const int32_t *col, int64_t E,
                          int64_t T, const int64_t *chain_row, U128 *rng, const float *Qdev, float *qpart,
                          const SampleConst &c, uint64_t N, int atomic_mode, hipStream_t st) {
-    constexpr int PF = 4;
     if (atomic_mode && d >= 1 && d <= 4) {
-        switch (d) {
-            case 1: k_sgd_epoch_hw<1, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N); break;
-            case 2: k_sgd_epoch_hw<2, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N); break;
-            case 3: k_sgd_epoch_hw<3, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N); break;
-            default: k_sgd_epoch_hw<4, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N); break;
+        const Tuning &tu = tuning();
+        SampleConst cc = c;
+        cc.fast_math = tu.sgd_fast_math;
+        switch (tu.sgd_prefetch) {
+            case 1: launch_hw<1>(d, grid, Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, cc, N, tu.sgd_plain_far, st); break;
+            case 2: launch_hw<2>(d, grid, Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, cc, N, tu.sgd_plain_far, st); break;
+            case 8: launch_hw<8>(d, grid, Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, cc, N, tu.sgd_plain_far, st); break;
+            default: launch_hw<4>(d, grid, Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, cc, N, tu.sgd_plain_far, st); break;
         }
         return;
     }
@@ -454,6 +494,7 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
         c.step = p.alpha * (1.f - fe * fe);
         c.noise = (uint32_t)p.n_noise[epoch];
         c.noise_f = (float)p.n_noise[epoch];
+        c.fast_math = 0;
         launch_epoch(d, grid, dY, d_row_ptr, d_col, E, T, chain_row.p, rng.p, qdev.p, qpart.p, c, (uint64_t)N,
                      p.atomic_updates, st);
         NCVIS_LAUNCH_CHECK();

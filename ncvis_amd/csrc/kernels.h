@@ -5,6 +5,20 @@
 
 namespace ncvis_hip {
 
+// Process-wide tuning knobs (ncvis_tune in the C ABI). Defaults are the measured best on MI355X;
+// none of them changes the result of a single-chain (n_threads = 1) fit.
+struct Tuning {
+    int sgd_prefetch = 1;            // rows fetched together per chain in the many-chain epoch (1,2,4,8)
+    int sgd_plain_far = 0;           // 1: far-endpoint updates with plain stores instead of atomics
+    int sgd_fast_math = 0;           // 1: single-precision powf/expf in the many-chain epoch
+    int64_t edges_per_chain = 64;    // auto chain count = E / edges_per_chain ...
+    int64_t max_chains = 1 << 18;    // ... capped here
+    int knn_dma = -1;                // kNN tiles fetch stages with LDS DMA (1), via registers (0), auto (-1)
+    int knn_select = -1;             // kNN top-k: register lists (0), buffered in HBM (1), auto (-1)
+    int knn_stage_mult = 1;          // kNN LDS stage rows x1 or x2 (register-staged kernel)
+};
+Tuning &tuning();
+
 // ---- kNN (knn.hip) ---------------------------------------------------------------------
 // Exact k-nearest-neighbour table in the reference's findKNN row format: for each row the k+1
 // smallest (distance, label) pairs are found, the smallest is dropped and the remaining k are

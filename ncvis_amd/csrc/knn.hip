@@ -155,7 +155,7 @@ __device__ __forceinline__ f32x16_t mfma_step(typename Frag<P>::T a, typename Fr
 
 // MFMA 32x32 output layout: lane l holds column l%32 and rows 8*(i/4) + 4*(l/32) + (i%4),
 // i = 0..15. Candidates are the A rows, queries the B columns, so a lane holds one query.
-template <int P, int QB, int KL, int DK>
+template <int P, int QB, int KL, int DK, int SM = 1>
 __global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_, const float *__restrict__ cn,
                                                    int64_t N, int64_t Npad, float beta,
                                                    int32_t *__restrict__ cand) {
@@ -164,13 +164,14 @@ __global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_,
     constexpr int DP = 32 * DK;                       // padded row length (elements)
     constexpr int KS = (P == PREC_FP32) ? DP / 2 : DP / 16;  // MFMA k-steps per row
     constexpr int ROWB = DP * ESZ;                    // row bytes in HBM
-    constexpr int CB = (ROWB >= 512) ? 32 : 64;       // candidate rows per LDS stage
+    constexpr int CB = ((ROWB >= 512) ? 32 : 64) * SM;  // candidate rows per LDS stage
     constexpr int LROWB = ROWB + 16;                  // padded LDS row (bank spread)
     constexpr int NV = CB * ROWB / 16 / 256;          // uint4 per thread per stage
     constexpr int CHUNKS = ROWB / 16;                 // uint4 per row
     static_assert(NV >= 1 && NV * 256 * 16 == CB * ROWB, "stage must split evenly over 256 threads");
 
     __shared__ __align__(16) unsigned char lds[2][CB * LROWB];
+    __shared__ __align__(16) float ldc[2][CB];  // the stage's score offsets travel with its rows
 
     const unsigned char *Xh = (const unsigned char *)Xh_;
     const int lane = threadIdx.x & 63;
@@ -213,15 +214,19 @@ __global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_,
     }
 
     const int64_t nstages = Npad / CB;
+    const bool cn_lane = threadIdx.x < CB / 4;  // these threads carry the stage's cn (float4 each)
     uint4 stage[NV];
+    float4 stage_cn = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(Xh + (int64_t)(v * 256 + threadIdx.x) * 16);
+    if (cn_lane) stage_cn = *(const float4 *)(cn + 4 * threadIdx.x);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         int u = v * 256 + threadIdx.x;
         int r = u / CHUNKS, c = u - r * CHUNKS;
         *(uint4 *)(&lds[0][r * LROWB + c * 16]) = stage[v];
     }
+    if (cn_lane) *(float4 *)(&ldc[0][4 * threadIdx.x]) = stage_cn;
     __syncthreads();
 
     for (int64_t st = 0; st < nstages; ++st) {
@@ -232,6 +237,7 @@ __global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_,
             const unsigned char *nb = Xh + nx * CB * (int64_t)ROWB;
 #pragma unroll
             for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(nb + (int64_t)(v * 256 + threadIdx.x) * 16);
+            if (cn_lane) stage_cn = *(const float4 *)(cn + nx * CB + 4 * threadIdx.x);
         }
 #pragma unroll
         for (int rb = 0; rb < CB / 32; ++rb) {
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_,
             float cnv[16];
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                float4 v = *(const float4 *)(cn + c0 + 8 * g + 4 * h);
+                float4 v = *(const float4 *)(&ldc[buf][rb * 32 + 8 * g + 4 * h]);
                 cnv[4 * g] = v.x;
                 cnv[4 * g + 1] = v.y;
                 cnv[4 * g + 2] = v.z;
@@ -298,9 +304,371 @@ __global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_,
                 int r = u / CHUNKS, c = u - r * CHUNKS;
                 *(uint4 *)(&lds[buf ^ 1][r * LROWB + c * 16]) = stage[v];
             }
+            if (cn_lane) *(float4 *)(&ldc[buf ^ 1][4 * threadIdx.x]) = stage_cn;
         }
         __syncthreads();
     }
+
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+        int64_t q = q0 + qb * 32 + col;
+        if (q < N) {
+            int32_t *o = cand + q * (2 * KL) + h * KL;
+#pragma unroll
+            for (int p = 0; p < KL; ++p) o[p] = li[qb][p];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------ buffered selection
+// Variant of k_knn_tiles whose hot loop holds no list: each (query, lane half) keeps only a
+// threshold and appends candidates that beat it, {score bits, label}, to its own buffer of CAP
+// entries in HBM (a few hundred appends over a whole sweep). When a buffer could overflow, the
+// wave re-selects the best KL of every buffer in lockstep (one sorted-insertion pass per buffered
+// slot, registers live only inside that pass) and tightens the thresholds. Entries are appended
+// in ascending label order and re-selected in buffer order with strict '<', so the final lists
+// equal one-at-a-time insertion.
+template <int KL>
+__device__ __forceinline__ void buf_reselect(uint2 *__restrict__ b, int &cnt, float &thr) {
+    float ld[KL];
+    int li[KL];
+#pragma unroll
+    for (int p = 0; p < KL; ++p) {
+        ld[p] = INFINITY;
+        li[p] = -1;
+    }
+    for (int j = 0; __any(j < cnt); ++j) {
+        const uint2 e = (j < cnt) ? b[j] : make_uint2(0x7f800000u, 0u);
+        list_insert<KL>(ld, li, __uint_as_float(e.x), (int)e.y);
+    }
+    int nvalid = 0;
+#pragma unroll
+    for (int p = 0; p < KL; ++p) {
+        b[p] = make_uint2(__float_as_uint(ld[p]), (unsigned)li[p]);
+        nvalid += (li[p] >= 0) ? 1 : 0;
+    }
+    cnt = nvalid;
+    thr = (nvalid == KL) ? ld[KL - 1] : INFINITY;
+}
+
+template <int P, int QB, int KL, int DK>
+__global__ __launch_bounds__(256) void k_knn_tiles_buf(const void *__restrict__ Xh_, const float *__restrict__ cn,
+                                                       int64_t N, int64_t Npad, float beta,
+                                                       uint2 *__restrict__ sbuf, int32_t *__restrict__ cand) {
+    using FT = typename Frag<P>::T;
+    constexpr int ESZ = Frag<P>::ESZ;
+    constexpr int DP = 32 * DK;
+    constexpr int KS = (P == PREC_FP32) ? DP / 2 : DP / 16;
+    constexpr int ROWB = DP * ESZ;
+    constexpr int CB = (ROWB >= 512) ? 32 : 64;
+    constexpr int LROWB = ROWB + 16;
+    constexpr int NV = CB * ROWB / 16 / 256;
+    constexpr int CHUNKS = ROWB / 16;
+    constexpr int CAP = 4 * KL;
+    static_assert(NV >= 1 && NV * 256 * 16 == CB * ROWB, "stage must split evenly over 256 threads");
+
+    __shared__ __align__(16) unsigned char lds[2][CB * LROWB];
+
+    const unsigned char *Xh = (const unsigned char *)Xh_;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int h = lane >> 5;
+    const int col = lane & 31;
+    const int64_t q0 = ((int64_t)blockIdx.x * 4 + wave) * (32 * QB);
+
+    FT qf[QB][KS];
+    uint2 *mb[QB];
+    float thr[QB];
+    int cnt[QB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+        const int64_t q = q0 + qb * 32 + col;
+        const unsigned char *row = Xh + q * (int64_t)ROWB;
+        if constexpr (P == PREC_FP32) {
+#pragma unroll
+            for (int s = 0; s < KS; s += 4) {
+                float4 v = *(const float4 *)(row + (h * (DP / 2) + s) * 4);
+                qf[qb][s] = v.x;
+                qf[qb][s + 1] = v.y;
+                qf[qb][s + 2] = v.z;
+                qf[qb][s + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) qf[qb][s] = *(const uint4 *)(row + s * 32 + h * 16);
+        }
+        mb[qb] = sbuf + (q * 2 + h) * (int64_t)CAP;
+        thr[qb] = INFINITY;
+        cnt[qb] = 0;
+    }
+
+    const int64_t nstages = Npad / CB;
+    uint4 stage[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(Xh + (int64_t)(v * 256 + threadIdx.x) * 16);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        int u = v * 256 + threadIdx.x;
+        int r = u / CHUNKS, c = u - r * CHUNKS;
+        *(uint4 *)(&lds[0][r * LROWB + c * 16]) = stage[v];
+    }
+    __syncthreads();
+
+    for (int64_t st = 0; st < nstages; ++st) {
+        const int buf = (int)(st & 1);
+        {
+            const int64_t nx = (st + 1 < nstages) ? st + 1 : st;
+            const unsigned char *nb = Xh + nx * CB * (int64_t)ROWB;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(nb + (int64_t)(v * 256 + threadIdx.x) * 16);
+        }
+#pragma unroll
+        for (int rb = 0; rb < CB / 32; ++rb) {
+            const int64_t c0 = st * CB + rb * 32;
+            const unsigned char *arow = &lds[buf][(rb * 32 + col) * LROWB];
+            f32x16_t acc[QB];
+#pragma unroll
+            for (int qb = 0; qb < QB; ++qb) acc[qb] = f32x16_t{};
+            if constexpr (P == PREC_FP32) {
+#pragma unroll
+                for (int s = 0; s < KS; s += 4) {
+                    float4 a4 = *(const float4 *)(arow + (h * (DP / 2) + s) * 4);
+#pragma unroll
+                    for (int qb = 0; qb < QB; ++qb) {
+                        acc[qb] = mfma_step<P>(a4.x, qf[qb][s], acc[qb]);
+                        acc[qb] = mfma_step<P>(a4.y, qf[qb][s + 1], acc[qb]);
+                        acc[qb] = mfma_step<P>(a4.z, qf[qb][s + 2], acc[qb]);
+                        acc[qb] = mfma_step<P>(a4.w, qf[qb][s + 3], acc[qb]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+                    uint4 a = *(const uint4 *)(arow + s * 32 + h * 16);
+#pragma unroll
+                    for (int qb = 0; qb < QB; ++qb) acc[qb] = mfma_step<P>(a, qf[qb][s], acc[qb]);
+                }
+            }
+            float cnv[16];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                float4 v = *(const float4 *)(cn + c0 + 8 * g + 4 * h);
+                cnv[4 * g] = v.x;
+                cnv[4 * g + 1] = v.y;
+                cnv[4 * g + 2] = v.z;
+                cnv[4 * g + 3] = v.w;
+            }
+#pragma unroll
+            for (int qb = 0; qb < QB; ++qb) {
+                float m = INFINITY;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) m = fminf(m, fmaf(beta, acc[qb][i], cnv[i]));
+                if (__any(m < thr[qb])) {
+                    if (__any(cnt[qb] > CAP - 16)) buf_reselect<KL>(mb[qb], cnt[qb], thr[qb]);
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const float sc = fmaf(beta, acc[qb][i], cnv[i]);
+                        if (sc < thr[qb]) {
+                            mb[qb][cnt[qb]] = make_uint2(__float_as_uint(sc), (unsigned)(c0 + 8 * (i >> 2) + 4 * h + (i & 3)));
+                            ++cnt[qb];
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            int u = v * 256 + threadIdx.x;
+            int r = u / CHUNKS, c = u - r * CHUNKS;
+            *(uint4 *)(&lds[buf ^ 1][r * LROWB + c * 16]) = stage[v];
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+        buf_reselect<KL>(mb[qb], cnt[qb], thr[qb]);
+        const int64_t q = q0 + qb * 32 + col;
+        if (q < N) {
+            int32_t *o = cand + q * (2 * KL) + h * KL;
+#pragma unroll
+            for (int p = 0; p < KL; ++p) o[p] = (int32_t)mb[qb][p].y;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------ LDS-DMA tile kernel
+// Same tile and selection as k_knn_tiles, but candidate stages are fetched with
+// global_load_lds_dwordx4 straight into LDS (no register staging), three stages in flight. LDS
+// rows are unpadded; the 16-byte chunk c of row r sits in slot c ^ (r % CHUNKS), so the
+// fragment reads of 32 consecutive rows at one chunk index hit distinct banks. Each wave keeps
+// its own copy of a stage's score offsets, fetched by its first CB/4 lanes.
+#define NCVIS_AS1(p) ((const void __attribute__((address_space(1))) *)(p))
+#define NCVIS_AS3(p) ((void __attribute__((address_space(3))) *)(p))
+
+template <int NV, int CB, int ROWB, int CHUNKS>
+__device__ __forceinline__ void dma_stage(const unsigned char *__restrict__ Xh, const float *__restrict__ cn,
+                                          int64_t stage, unsigned char *lbuf, float *lcn_wave, int wave, int lane) {
+    const unsigned char *base = Xh + stage * (int64_t)(CB * ROWB);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const int p = (v * 4 + wave) * 64 + lane;  // 16-byte LDS slot this lane fills
+        const int r = p / CHUNKS, s = p % CHUNKS;
+        const int c = s ^ (r & (CHUNKS - 1));
+        __builtin_amdgcn_global_load_lds(NCVIS_AS1(base + r * ROWB + c * 16), NCVIS_AS3(lbuf + (v * 4 + wave) * 1024),
+                                         16, 0, 0);
+    }
+    if (lane < CB / 4)
+        __builtin_amdgcn_global_load_lds(NCVIS_AS1(cn + stage * CB + 4 * lane), NCVIS_AS3(lcn_wave), 16, 0, 0);
+}
+
+// s_waitcnt with only the vector-memory counter constrained (gfx9 encoding)
+#define NCVIS_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
+
+template <int P, int QB, int KL, int DK>
+__global__ __launch_bounds__(256) void k_knn_tiles_dma(const void *__restrict__ Xh_, const float *__restrict__ cn,
+                                                       int64_t N, int64_t Npad, float beta,
+                                                       int32_t *__restrict__ cand) {
+    using FT = typename Frag<P>::T;
+    constexpr int ESZ = Frag<P>::ESZ;
+    constexpr int DP = 32 * DK;
+    constexpr int KS = (P == PREC_FP32) ? DP / 2 : DP / 16;
+    constexpr int ROWB = DP * ESZ;
+    constexpr int CB = (ROWB >= 512) ? 32 : 64;
+    constexpr int CHUNKS = ROWB / 16;
+    constexpr int NV = CB * ROWB / 16 / 256;
+    constexpr int SB = CB * ROWB;
+    static_assert(NV >= 1 && NV * 4096 == SB, "stage must split into whole wave transfers");
+    static_assert((CHUNKS & (CHUNKS - 1)) == 0, "swizzle needs a power-of-two chunk count");
+
+    __shared__ __align__(16) unsigned char lds0[SB];
+    __shared__ __align__(16) unsigned char lds1[SB];
+    __shared__ __align__(16) unsigned char lds2[SB];
+    __shared__ __align__(16) float ldc0[4][CB];
+    __shared__ __align__(16) float ldc1[4][CB];
+    __shared__ __align__(16) float ldc2[4][CB];
+
+    const unsigned char *Xh = (const unsigned char *)Xh_;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int h = lane >> 5;
+    const int col = lane & 31;
+    const int64_t q0 = ((int64_t)blockIdx.x * 4 + wave) * (32 * QB);
+
+    FT qf[QB][KS];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+        const unsigned char *row = Xh + (q0 + qb * 32 + col) * (int64_t)ROWB;
+        if constexpr (P == PREC_FP32) {
+#pragma unroll
+            for (int s = 0; s < KS; s += 4) {
+                float4 v = *(const float4 *)(row + (h * (DP / 2) + s) * 4);
+                qf[qb][s] = v.x;
+                qf[qb][s + 1] = v.y;
+                qf[qb][s + 2] = v.z;
+                qf[qb][s + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) qf[qb][s] = *(const uint4 *)(row + s * 32 + h * 16);
+        }
+    }
+    float ld[QB][KL];
+    int li[QB][KL];
+    float thr[QB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+#pragma unroll
+        for (int p = 0; p < KL; ++p) {
+            ld[qb][p] = INFINITY;
+            li[qb][p] = -1;
+        }
+        thr[qb] = INFINITY;
+    }
+
+    const int64_t nstages = Npad / CB;
+    const int64_t last = nstages - 1;
+    const int swz = col & (CHUNKS - 1);  // rows rb*32 + col share col's swizzle (CHUNKS <= 32 divides 32)
+
+#define NCVIS_DMA(STG, LB, LC) \
+    dma_stage<NV, CB, ROWB, CHUNKS>(Xh, cn, (STG) < last ? (STG) : last, LB, LC[wave], wave, lane)
+
+#define NCVIS_DMA_COMPUTE(LB, LC, ST)                                                                  \
+    do {                                                                                               \
+        _Pragma("unroll") for (int rb = 0; rb < CB / 32; ++rb) {                                       \
+            const int64_t c0 = (ST) * CB + rb * 32;                                                    \
+            const unsigned char *arow = LB + (rb * 32 + col) * ROWB;                                   \
+            f32x16_t acc[QB];                                                                          \
+            _Pragma("unroll") for (int qb = 0; qb < QB; ++qb) acc[qb] = f32x16_t{};                    \
+            if constexpr (P == PREC_FP32) {                                                            \
+                _Pragma("unroll") for (int s = 0; s < KS; s += 4) {                                    \
+                    const int ch = (h * (DP / 2) + s) / 4;                                             \
+                    float4 a4 = *(const float4 *)(arow + ((ch ^ swz) * 16));                          \
+                    _Pragma("unroll") for (int qb = 0; qb < QB; ++qb) {                                \
+                        acc[qb] = mfma_step<P>(a4.x, qf[qb][s], acc[qb]);                              \
+                        acc[qb] = mfma_step<P>(a4.y, qf[qb][s + 1], acc[qb]);                          \
+                        acc[qb] = mfma_step<P>(a4.z, qf[qb][s + 2], acc[qb]);                          \
+                        acc[qb] = mfma_step<P>(a4.w, qf[qb][s + 3], acc[qb]);                          \
+                    }                                                                                  \
+                }                                                                                      \
+            } else {                                                                                   \
+                _Pragma("unroll") for (int s = 0; s < KS; ++s) {                                       \
+                    uint4 a = *(const uint4 *)(arow + (((2 * s + h) ^ swz) * 16));                    \
+                    _Pragma("unroll") for (int qb = 0; qb < QB; ++qb) acc[qb] =                        \
+                        mfma_step<P>(a, qf[qb][s], acc[qb]);                                           \
+                }                                                                                      \
+            }                                                                                          \
+            float cnv[16];                                                                             \
+            _Pragma("unroll") for (int g = 0; g < 4; ++g) {                                            \
+                float4 v = *(const float4 *)(&LC[wave][rb * 32 + 8 * g + 4 * h]);                      \
+                cnv[4 * g] = v.x;                                                                      \
+                cnv[4 * g + 1] = v.y;                                                                  \
+                cnv[4 * g + 2] = v.z;                                                                  \
+                cnv[4 * g + 3] = v.w;                                                                  \
+            }                                                                                          \
+            _Pragma("unroll") for (int qb = 0; qb < QB; ++qb) {                                        \
+                float m = INFINITY;                                                                    \
+                _Pragma("unroll") for (int i = 0; i < 16; ++i) m = fminf(m, fmaf(beta, acc[qb][i], cnv[i])); \
+                if (__any(m < thr[qb])) {                                                              \
+                    _Pragma("unroll") for (int i = 0; i < 16; ++i) {                                   \
+                        float sc = fmaf(beta, acc[qb][i], cnv[i]);                                     \
+                        bool pass = sc < thr[qb];                                                      \
+                        if (__any(pass)) {                                                             \
+                            if (pass) {                                                                \
+                                int ci = (int)(c0 + 8 * (i >> 2) + 4 * h + (i & 3));                   \
+                                list_insert<KL>(ld[qb], li[qb], sc, ci);                               \
+                                thr[qb] = ld[qb][KL - 1];                                              \
+                            }                                                                          \
+                        }                                                                              \
+                    }                                                                                  \
+                }                                                                                      \
+            }                                                                                          \
+        }                                                                                              \
+    } while (0)
+
+    // one step: wait for stage ST (only the next stage's transfers may still be in flight),
+    // barrier, start stage ST+2 into the buffer stage ST-1 used, then compute stage ST
+#define NCVIS_DMA_STEP(ST, LB, LC, NLB, NLC)                                                           \
+    do {                                                                                               \
+        NCVIS_WAIT_VMCNT(NV + 1);                                                                      \
+        __syncthreads();                                                                               \
+        NCVIS_DMA((ST) + 2, NLB, NLC);                                                                 \
+        NCVIS_DMA_COMPUTE(LB, LC, ST);                                                                 \
+    } while (0)
+
+    NCVIS_DMA(0, lds0, ldc0);
+    NCVIS_DMA(1, lds1, ldc1);
+    for (int64_t st = 0; st < nstages; st += 3) {
+        NCVIS_DMA_STEP(st, lds0, ldc0, lds2, ldc2);
+        if (st + 1 >= nstages) break;
+        NCVIS_DMA_STEP(st + 1, lds1, ldc1, lds0, ldc0);
+        if (st + 2 >= nstages) break;
+        NCVIS_DMA_STEP(st + 2, lds2, ldc2, lds1, ldc1);
+    }
+    NCVIS_WAIT_VMCNT(0);
+#undef NCVIS_DMA_STEP
+#undef NCVIS_DMA_COMPUTE
+#undef NCVIS_DMA
 
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb) {
@@ -452,6 +820,42 @@ __global__ __launch_bounds__(256) void k_knn_generic(const float *__restrict__ X
 template <int P, int QB, int KL>
 static void launch_tiles_dk(int DK, dim3 grid, const void *Xh, const float *cn, int64_t N, int64_t Npad,
                             float beta, int32_t *cand, hipStream_t st) {
+    const Tuning &tu = tuning();
+    const int select = tu.knn_select >= 0 ? tu.knn_select : 0;
+    const int dma = tu.knn_dma >= 0 ? tu.knn_dma : (P == PREC_FP32 ? 1 : 0);
+    if (select == 1) {
+        DevBuf<uint2> sbuf((size_t)Npad * 2 * 4 * KL);
+        switch (DK) {
+            case 1: k_knn_tiles_buf<P, QB, KL, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, sbuf.p, cand); break;
+            case 2: k_knn_tiles_buf<P, QB, KL, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, sbuf.p, cand); break;
+            case 4: k_knn_tiles_buf<P, QB, KL, 4><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, sbuf.p, cand); break;
+            case 8: k_knn_tiles_buf<P, QB, KL, 8><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, sbuf.p, cand); break;
+            default: throw HipError("[ncvis_amd] unsupported padded dimension");
+        }
+        NCVIS_LAUNCH_CHECK();
+        NCVIS_HIP_CHECK(hipStreamSynchronize(st));  // sbuf is released on return
+        return;
+    }
+    if (dma) {
+        switch (DK) {
+            case 1: k_knn_tiles_dma<P, QB, KL, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            case 2: k_knn_tiles_dma<P, QB, KL, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            case 4: k_knn_tiles_dma<P, QB, KL, 4><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            case 8: k_knn_tiles_dma<P, QB, KL, 8><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            default: throw HipError("[ncvis_amd] unsupported padded dimension");
+        }
+        return;
+    }
+    if (tu.knn_stage_mult == 2) {
+        switch (DK) {
+            case 1: k_knn_tiles<P, QB, KL, 1, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            case 2: k_knn_tiles<P, QB, KL, 2, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            case 4: k_knn_tiles<P, QB, KL, 4, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            case 8: k_knn_tiles<P, QB, KL, 8, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            default: throw HipError("[ncvis_amd] unsupported padded dimension");
+        }
+        return;
+    }
     switch (DK) {
         case 1: k_knn_tiles<P, QB, KL, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
         case 2: k_knn_tiles<P, QB, KL, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
