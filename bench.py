@@ -52,6 +52,20 @@ def make_data(w, seed=0):
     return X
 
 
+def measured_traffic(workload_desc, precision):
+    """HBM bytes per launch of the two dominant kernels, from the calibrated FETCH_SIZE/WRITE_SIZE
+    counter runs committed for the 1M x 128 workload; {} for other workloads or precisions."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_fit_1m_d128.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except OSError:
+        return {}
+    if t.get("workload") != workload_desc or precision != "fp16":
+        return {}
+    return {name: float(v["traffic_bytes"]) for name, v in t["kernels"].items()}
+
+
 def recall_at_k(torch, dX, inds, k, queries):
     """recall@k of the GPU table against float64 exact search (same self-drop convention)."""
     X64 = dX.double()
@@ -191,17 +205,20 @@ def main():
         torch.cuda.synchronize()
         fp32["recall@15"] = recall_at_k(torch, dX, inds, k, queries)
 
-    # roofline of the dominant kernel
+    # roofline of the dominant kernel: algorithmic work per launch / event-timed launch duration
     n_samples = E * float(np.sum(1 + plan))
     sgd_bytes = E * 8.0 * n_epochs + n_samples * 4 * d * 4
     knn_flops = 2.0 * N * N * D
     knn_tiles_s = stage["knn_tiles"] / 1e3
-    sgd_s = stage["optimize"] / 1e3
+    sgd_s = stage["sgd_kernel"] / 1e3
+    traffic = measured_traffic(w["desc"], args.precision)
     roof_knn = {"bound": "mfma", "kernel": "k_knn_tiles", "achieved": knn_flops / knn_tiles_s / 1e12,
-                "peak": MFMA_PEAK_TFLOPS[args.precision], "unit": "TFLOP/s", "traffic": None}
+                "peak": MFMA_PEAK_TFLOPS[args.precision], "unit": "TFLOP/s",
+                "traffic": traffic.get("k_knn_tiles"), "launch_ms": stage["knn_tiles"]}
     roof_knn["frac"] = roof_knn["achieved"] / roof_knn["peak"]
     roof_sgd = {"bound": "hbm", "kernel": "k_sgd_epoch", "achieved": sgd_bytes / sgd_s / 1e9,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None}
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": traffic.get("k_sgd_epoch"),
+                "launch_ms": stage["sgd_kernel"] / n_epochs}
     roof_sgd["frac"] = roof_sgd["achieved"] / roof_sgd["peak"]
     dominant = roof_knn if knn_tiles_s >= sgd_s else roof_sgd
 
@@ -239,7 +256,7 @@ def main():
         "fp32": {k_: (round(v, 5) if isinstance(v, float) else v) for k_, v in fp32.items()},
         "stage_ms": {k_: round(v, 2) for k_, v in stage.items()},
         "n_edges": E,
-        "edge_updates_per_s": n_samples / sgd_s,
+        "edge_updates_per_s": n_samples / (stage["optimize"] / 1e3),
         "roofline": {k_: (round(v, 5) if isinstance(v, float) else v) for k_, v in dominant.items()},
         "roofline_all": [{k_: (round(v, 5) if isinstance(v, float) else v) for k_, v in r.items()}
                          for r in (roof_knn, roof_sgd)],

@@ -60,9 +60,11 @@ const char *ncvis_last_error(const ncvis_handle *h);
  *   "chains"          override the SGD chain count (0 = from n_threads) */
 int ncvis_set_option(ncvis_handle *h, const char *key, double value);
 
-/* Process-wide tuning knobs (no reference counterpart; used by the autotuner). None changes a
- * single-chain fit. Keys: "sgd_prefetch" (1,2,4,8), "sgd_plain_far" (0/1), "sgd_fast_math"
- * (0/1), "edges_per_chain" (auto chain count = E / this), "max_chains". */
+/* Process-wide tuning knobs (no reference counterpart). None changes a single-chain fit; the kNN
+ * knobs never change the kNN table. Keys: "sgd_prefetch" (1,2,4,8), "sgd_plain_far" (0/1),
+ * "sgd_fast_math" (0/1), "edges_per_chain" (auto chain count = E / this), "max_chains",
+ * "knn_dma" (-1 auto, 0, 1), "knn_select" (-1 auto, 0 register lists, 1 buffered),
+ * "knn_stage_mult" (1, 2), "knn_skew" (0/1: skewed two-set 16-bit tile kernel). */
 int ncvis_tune(const char *key, double value);
 
 /* ncvis::NCVis::fit_transform: X [N, D] and Y [N, d] are host, row-major float32. */
@@ -72,8 +74,9 @@ int ncvis_fit_transform(ncvis_handle *h, const float *X, size_t N, size_t D, flo
 int ncvis_fit_transform_device(ncvis_handle *h, const float *dX, size_t N, size_t D, float *dY, void *stream);
 
 /* Stage times of the last fit (milliseconds): [knn, symmetrize, init, optimize, total, knn_tiles,
- * knn_rerank], the edge count E, the final Q and the effective chain count. */
-int ncvis_get_stats(const ncvis_handle *h, double *stage_ms7, int64_t *n_edges, float *final_Q, int64_t *chains);
+ * knn_rerank, sgd_kernel] (sgd_kernel: the epoch kernels alone, summed over epochs), the edge
+ * count E, the final Q and the effective chain count. */
+int ncvis_get_stats(const ncvis_handle *h, double *stage_ms8, int64_t *n_edges, float *final_Q, int64_t *chains);
 
 /* ---- stage entry points (device pointers) ---- */
 const char *ncvis_get_last_error(void);

@@ -6,6 +6,9 @@ the Cython extension ``_ncvis`` (C++ ``ncvis::NCVis`` class); build them with
 ``python -m ncvis_amd.build``. There is no CPU fallback: without the built extension, or without a
 GPU, the API raises.
 """
+from ._runtime import share_torch_hip_runtime as _share_rt
+
+_share_rt()
 try:
     from ._ncvis import NCVis, NCVisWrapper, find_ab_params  # noqa: F401
 except ImportError as _e:  # pragma: no cover - exercised only when the extension is missing

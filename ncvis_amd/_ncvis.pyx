@@ -97,11 +97,12 @@ cdef class NCVisWrapper:
         self.c_ncvis.set_option(key.encode(), value)
 
     def stats(self):
-        cdef double ms[7]
+        cdef double ms[8]
         cdef long long e = 0, c = 0
         cdef float q = 0
         self.c_ncvis.stats(ms, &e, &q, &c)
-        names = ("knn", "symmetrize", "init", "optimize", "total", "knn_tiles", "knn_rerank")
+        names = ("knn", "symmetrize", "init", "optimize", "total", "knn_tiles", "knn_rerank",
+                 "sgd_kernel")
         return {"stage_ms": {n: ms[i] for i, n in enumerate(names)}, "n_edges": e, "Q": q, "chains": c}
 
     def fit_transform(self, float[:, ::1] X, float[:, ::1] Y):

@@ -31,6 +31,8 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built (python -m ncvis_amd.build)")
+        from ._runtime import share_torch_hip_runtime
+        share_torch_hip_runtime()
         L = C.CDLL(LIB_PATH)
         L.ncvis_create.argtypes = [C.c_size_t] * 6 + [C.c_int, C.c_int] + [C.c_float] * 4 + [
             C.POINTER(C.c_size_t), C.c_int, C.POINTER(vp)]
@@ -152,10 +154,11 @@ class Handle:
                                                Y.ctypes.data))
 
     def stats(self):
-        ms = (C.c_double * 7)()
+        ms = (C.c_double * 8)()
         e, c, q = C.c_int64(), C.c_int64(), C.c_float()
         lib().ncvis_get_stats(self._h, ms, C.byref(e), C.byref(q), C.byref(c))
-        names = ("knn", "symmetrize", "init", "optimize", "total", "knn_tiles", "knn_rerank")
+        names = ("knn", "symmetrize", "init", "optimize", "total", "knn_tiles", "knn_rerank",
+                 "sgd_kernel")
         return {"stage_ms": {n: ms[i] for i, n in enumerate(names)}, "n_edges": e.value,
                 "Q": q.value, "chains": c.value}
 

@@ -24,7 +24,7 @@ struct ncvis_handle {
     std::string last_error;
     hipStream_t stream = nullptr;
     // stats of the last fit
-    double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+    double stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int64_t n_edges = 0;
     float final_Q = 0.f;
     int64_t chains_used = 0;
@@ -160,6 +160,8 @@ int ncvis_tune(const char *key, double value) {
             t.edges_per_chain = (int64_t)value;
         } else if (k == "knn_dma") {
             t.knn_dma = value < 0 ? -1 : (value != 0);
+        } else if (k == "knn_skew") {
+            t.knn_skew = value != 0 ? 1 : 0;
         } else if (k == "knn_stage_mult") {
             if (value != 1 && value != 2) throw std::invalid_argument("[ncvis_amd] knn_stage_mult: 1 or 2");
             t.knn_stage_mult = (int)value;
@@ -227,7 +229,8 @@ int ncvis_fit_transform_device(ncvis_handle *h, const float *dX, size_t N, size_
         p.chains = T;
         p.atomic_updates = h->atomic_updates >= 0 ? h->atomic_updates : (T > 1 ? 1 : 0);
         float Q = 0.f;  // ncvis.cpp:417
-        optimize_run(n, (int)h->d, row_ptr.p, col.p, E, dY, &Q, p, st, nullptr);
+        double epoch_kernel_ms = 0.0;
+        optimize_run(n, (int)h->d, row_ptr.p, col.p, E, dY, &Q, p, st, nullptr, &epoch_kernel_ms);
         NCVIS_HIP_CHECK(hipEventRecord(ev[4], st));
         NCVIS_HIP_CHECK(hipEventSynchronize(ev[4]));
         float ms[4];
@@ -236,6 +239,7 @@ int ncvis_fit_transform_device(ncvis_handle *h, const float *dX, size_t N, size_
         h->stage_ms[4] = ms[0] + ms[1] + ms[2] + ms[3];
         h->stage_ms[5] = ks.tile_ms;
         h->stage_ms[6] = ks.rerank_ms;
+        h->stage_ms[7] = epoch_kernel_ms;
         h->n_edges = E;
         h->final_Q = Q;
         h->chains_used = T;
@@ -267,10 +271,10 @@ int ncvis_fit_transform(ncvis_handle *h, const float *X, size_t N, size_t D, flo
     });
 }
 
-int ncvis_get_stats(const ncvis_handle *h, double *stage_ms7, int64_t *n_edges, float *final_Q, int64_t *chains) {
+int ncvis_get_stats(const ncvis_handle *h, double *stage_ms8, int64_t *n_edges, float *final_Q, int64_t *chains) {
     if (!h) return NCVIS_EINVAL;
-    if (stage_ms7)
-        for (int i = 0; i < 7; ++i) stage_ms7[i] = h->stage_ms[i];
+    if (stage_ms8)
+        for (int i = 0; i < 8; ++i) stage_ms8[i] = h->stage_ms[i];
     if (n_edges) *n_edges = h->n_edges;
     if (final_Q) *final_Q = h->final_Q;
     if (chains) *chains = h->chains_used;

@@ -21,6 +21,7 @@
 //   k_nce_loss     per edge: positive + n explicit negatives; loss and dL/dQ as FP64 partial
 //                  sums, dL/dY scattered with FP32 atomics.
 #include <cfloat>
+#include <vector>
 
 #include "common.h"
 #include "kernels.h"
@@ -419,19 +420,28 @@ __global__ __launch_bounds__(256) void k_sgd_epoch_hw(float *__restrict__ Y, con
     qpart[t] = Qc;
 }
 
-__global__ __launch_bounds__(256) void k_q_reduce(const float *__restrict__ qpart, int64_t T, float *__restrict__ Qdev,
-                                                  float *__restrict__ trace, int epoch) {
-    __shared__ double red[256];
-    double s = 0.0;
-    for (int64_t t = threadIdx.x; t < T; t += 256) s += (double)qpart[t];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
+// One block of 1024 threads; each thread keeps four loads in flight so the 1 MB of chain copies at
+// 2^18 chains is read at bandwidth instead of one dependent load per thread per step.
+__global__ __launch_bounds__(1024) void k_q_reduce(const float *__restrict__ qpart, int64_t T,
+                                                   float *__restrict__ Qdev, float *__restrict__ trace, int epoch) {
+    __shared__ double red[1024 / 64];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int64_t t = threadIdx.x;
+    for (; t + 3 * 1024 < T; t += 4 * 1024) {
+        s0 += (double)qpart[t];
+        s1 += (double)qpart[t + 1024];
+        s2 += (double)qpart[t + 2 * 1024];
+        s3 += (double)qpart[t + 3 * 1024];
     }
+    for (; t < T; t += 1024) s0 += (double)qpart[t];
+    double s = (s0 + s1) + (s2 + s3);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        float q = (T == 1) ? qpart[0] : (float)(red[0] / (double)T);
+        double tot = 0.0;
+        for (int w = 0; w < 1024 / 64; ++w) tot += red[w];
+        float q = (T == 1) ? qpart[0] : (float)(tot / (double)T);
         *Qdev = q;
         trace[epoch] = q;
     }
@@ -474,7 +484,7 @@ static void launch_epoch(int d, dim3 grid, float *Y, const int64_t *row_ptr, con
 }
 
 void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t E, float *dY, float *Q,
-                  const SgdParams &p, hipStream_t st, float *q_trace) {
+                  const SgdParams &p, hipStream_t st, float *q_trace, double *epoch_kernel_ms) {
     if (N < 2 || d < 1) throw HipError("[ncvis_amd] optimize: invalid sizes");
     const int64_t T = p.chains < 1 ? 1 : p.chains;
     DevBuf<int64_t> chain_row((size_t)T);
@@ -485,6 +495,21 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
     NCVIS_LAUNCH_CHECK();
     NCVIS_HIP_CHECK(hipMemcpyAsync(qdev.p, Q, sizeof(float), hipMemcpyHostToDevice, st));
     dim3 grid((unsigned)cdiv(T, 256));
+    // Optional per-epoch kernel timing: one event pair around each epoch kernel on `st`.
+    std::vector<hipEvent_t> ev;
+    struct EvGuard {
+        std::vector<hipEvent_t> &e;
+        ~EvGuard() {
+            for (hipEvent_t x : e) (void)hipEventDestroy(x);
+        }
+    } evg{ev};
+    if (epoch_kernel_ms) {
+        ev.resize(2 * (size_t)std::max(p.n_epochs, 0));
+        for (auto &x : ev) {
+            x = nullptr;
+            NCVIS_HIP_CHECK(hipEventCreate(&x));
+        }
+    }
     for (int epoch = 0; epoch < p.n_epochs; ++epoch) {
         SampleConst c;
         c.a = p.a;
@@ -495,10 +520,12 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
         c.noise = (uint32_t)p.n_noise[epoch];
         c.noise_f = (float)p.n_noise[epoch];
         c.fast_math = 0;
+        if (epoch_kernel_ms) NCVIS_HIP_CHECK(hipEventRecord(ev[2 * epoch], st));
         launch_epoch(d, grid, dY, d_row_ptr, d_col, E, T, chain_row.p, rng.p, qdev.p, qpart.p, c, (uint64_t)N,
                      p.atomic_updates, st);
         NCVIS_LAUNCH_CHECK();
-        k_q_reduce<<<1, 256, 0, st>>>(qpart.p, T, qdev.p, qdev.p + 1, epoch);
+        if (epoch_kernel_ms) NCVIS_HIP_CHECK(hipEventRecord(ev[2 * epoch + 1], st));
+        k_q_reduce<<<1, 1024, 0, st>>>(qpart.p, T, qdev.p, qdev.p + 1, epoch);
         NCVIS_LAUNCH_CHECK();
     }
     NCVIS_HIP_CHECK(hipMemcpyAsync(Q, qdev.p, sizeof(float), hipMemcpyDeviceToHost, st));
@@ -506,6 +533,15 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
         NCVIS_HIP_CHECK(
             hipMemcpyAsync(q_trace, qdev.p + 1, sizeof(float) * p.n_epochs, hipMemcpyDeviceToHost, st));
     NCVIS_HIP_CHECK(hipStreamSynchronize(st));
+    if (epoch_kernel_ms) {
+        double tot = 0.0;
+        for (int epoch = 0; epoch < p.n_epochs; ++epoch) {
+            float ms = 0.f;
+            NCVIS_HIP_CHECK(hipEventElapsedTime(&ms, ev[2 * epoch], ev[2 * epoch + 1]));
+            tot += ms;
+        }
+        *epoch_kernel_ms = tot;
+    }
 }
 
 // ------------------------------------------------------------------------ NCE loss

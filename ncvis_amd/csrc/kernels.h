@@ -15,7 +15,8 @@ struct Tuning {
     int64_t max_chains = 1 << 18;    // ... capped here
     int knn_dma = -1;                // kNN tiles fetch stages with LDS DMA (1), via registers (0), auto (-1)
     int knn_select = -1;             // kNN top-k: register lists (0), buffered in HBM (1), auto (-1)
-    int knn_stage_mult = 1;          // kNN LDS stage rows x1 or x2 (register-staged kernel)
+    int knn_stage_mult = 2;          // kNN LDS stage rows x1 or x2 (register-staged kernels)
+    int knn_skew = 1;                // 16-bit kNN: skewed two-set tile kernel (1) or plain (0)
 };
 Tuning &tuning();
 
@@ -51,9 +52,11 @@ struct SgdParams {
 void init_embedding_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, float alpha,
                         int n_init_epochs, uint64_t seed, int64_t chains, float *dY, hipStream_t st);
 
-// Runs all epochs; Q (host in/out). `q_trace` (optional, host [n_epochs]) receives Q per epoch.
+// Runs all epochs; Q (host in/out). `q_trace` (optional, host [n_epochs]) receives Q per epoch;
+// `epoch_kernel_ms` (optional, host) the summed event time of the epoch kernels alone.
 void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t E, float *dY,
-                  float *Q, const SgdParams &p, hipStream_t st, float *q_trace = nullptr);
+                  float *Q, const SgdParams &p, hipStream_t st, float *q_trace = nullptr,
+                  double *epoch_kernel_ms = nullptr);
 
 // NCE loss and adjoint over an explicit negative table neg[E, n] (int32). dY (float, [N,d],
 // accumulated) and dQ / loss (double) are device pointers; dY may be null for loss only.

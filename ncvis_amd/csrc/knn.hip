@@ -106,20 +106,24 @@ __global__ void k_pack(const float *__restrict__ Xp, int64_t N, int64_t Npad, in
 }
 
 // ------------------------------------------------------------------------ top-k lists
-// Sorted ascending list; strict '<' keeps the earlier (smaller-label) entry first among equal
-// scores because every lane visits its candidates in ascending label order.
+// Sorted ascending list: x goes to the first slot p with x < ld[p], the tail shifts down one slot
+// and the last entry drops out. Among equal scores the earlier entry stays first and the later
+// one drops first, so with candidates visited in ascending label order a list holds the KL
+// smallest (score, label) pairs. Every new slot is a function of the old list only: the score
+// of slot p is med3(x, ld[p-1], ld[p]) (the list is sorted), its label comes from slot p-1, x or
+// itself, so the update runs in place without swaps. Requires x < ld[KL-1] (x not NaN).
 template <int KL>
 __device__ __forceinline__ void list_insert(float (&ld)[KL], int (&li)[KL], float x, int xi) {
+    bool c[KL];
 #pragma unroll
-    for (int p = 0; p < KL; ++p) {
-        bool sw = x < ld[p];
-        float td = ld[p];
-        int ti = li[p];
-        ld[p] = sw ? x : td;
-        li[p] = sw ? xi : ti;
-        x = sw ? td : x;
-        xi = sw ? ti : xi;
+    for (int p = 0; p < KL; ++p) c[p] = x < ld[p];
+#pragma unroll
+    for (int p = KL - 1; p >= 1; --p) {
+        ld[p] = __builtin_amdgcn_fmed3f(x, ld[p - 1], ld[p]);
+        li[p] = c[p - 1] ? li[p - 1] : (c[p] ? xi : li[p]);
     }
+    ld[0] = c[0] ? x : ld[0];
+    li[0] = c[0] ? xi : li[0];
 }
 
 template <int P>
@@ -307,6 +311,173 @@ __global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_,
             if (cn_lane) *(float4 *)(&ldc[buf ^ 1][4 * threadIdx.x]) = stage_cn;
         }
         __syncthreads();
+    }
+
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+        int64_t q = q0 + qb * 32 + col;
+        if (q < N) {
+            int32_t *o = cand + q * (2 * KL) + h * KL;
+#pragma unroll
+            for (int p = 0; p < KL; ++p) o[p] = li[qb][p];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------ skewed two-set tiles
+// Scores of one 32x32 block (held in the accumulator, overwritten in place by cn + beta * dot)
+// filtered into a lane's sorted list; candidates are visited in ascending label order.
+template <int KL>
+__device__ __forceinline__ void block_insert(float (&ld)[KL], int (&li)[KL], float &thr, const f32x16_t &sc,
+                                             int64_t c0, int h) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const bool pass = sc[i] < thr;
+        if (__any(pass)) {
+            if (pass) {
+                list_insert<KL>(ld, li, sc[i], (int)(c0 + 8 * (i >> 2) + 4 * h + (i & 3)));
+                thr = ld[KL - 1];
+            }
+        }
+    }
+}
+
+// cn + beta * dot for the lane's 16 rows of a block (offsets read from the stage's LDS copy);
+// returns the lane's smallest score.
+__device__ __forceinline__ float block_scores(f32x16_t &acc, const float *pc, int h, float beta) {
+    float m = INFINITY;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const float4 v = *(const float4 *)(pc + 8 * g + 4 * h);
+        acc[4 * g] = fmaf(beta, acc[4 * g], v.x);
+        acc[4 * g + 1] = fmaf(beta, acc[4 * g + 1], v.y);
+        acc[4 * g + 2] = fmaf(beta, acc[4 * g + 2], v.z);
+        acc[4 * g + 3] = fmaf(beta, acc[4 * g + 3], v.w);
+        m = fminf(m, fminf(fminf(acc[4 * g], acc[4 * g + 1]), fminf(acc[4 * g + 2], acc[4 * g + 3])));
+    }
+    return m;
+}
+
+// Same tiles, lists and visiting order as k_knn_tiles at QB = 2 (16-bit GEMM), with the two query
+// sets of a wave skewed by one block: the MFMA chain of set 0 for block b is issued together
+// with the scoring of set 1 for block b-1, and the chain of set 1 for block b together with the
+// scoring of set 0 for block b, so one wave keeps its MFMA pipe busy while its VALU filters the
+// other set. Score offsets are triple-buffered in LDS because set 1 reads the previous stage's
+// offsets after the stage barrier. The candidate lists equal k_knn_tiles' bit for bit.
+template <int P, int KL, int DK, int SM>
+__global__ __launch_bounds__(256) void k_knn_tiles_sk(const void *__restrict__ Xh_, const float *__restrict__ cn,
+                                                      int64_t N, int64_t Npad, float beta,
+                                                      int32_t *__restrict__ cand) {
+    static_assert(P != PREC_FP32, "16-bit candidate GEMM only");
+    constexpr int QB = 2;
+    constexpr int ESZ = 2;
+    constexpr int DP = 32 * DK;
+    constexpr int KS = DP / 16;
+    constexpr int ROWB = DP * ESZ;
+    constexpr int CB = ((ROWB >= 512) ? 32 : 64) * SM;
+    constexpr int LROWB = ROWB + 16;
+    constexpr int NV = CB * ROWB / 16 / 256;
+    constexpr int CHUNKS = ROWB / 16;
+    constexpr int NB = CB / 32;
+    static_assert(NV >= 1 && NV * 256 * 16 == CB * ROWB, "stage must split evenly over 256 threads");
+
+    __shared__ __align__(16) unsigned char lds[2][CB * LROWB];
+    __shared__ __align__(16) float ldc[3][CB];
+    __shared__ __align__(16) float ldinf[32];
+
+    const unsigned char *Xh = (const unsigned char *)Xh_;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int h = lane >> 5;
+    const int col = lane & 31;
+    const int64_t q0 = ((int64_t)blockIdx.x * 4 + wave) * (32 * QB);
+
+    uint4 qf[QB][KS];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+        const unsigned char *row = Xh + (q0 + qb * 32 + col) * (int64_t)ROWB;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) qf[qb][s] = *(const uint4 *)(row + s * 32 + h * 16);
+    }
+    float ld[QB][KL];
+    int li[QB][KL];
+    float thr[QB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+#pragma unroll
+        for (int p = 0; p < KL; ++p) {
+            ld[qb][p] = INFINITY;
+            li[qb][p] = -1;
+        }
+        thr[qb] = INFINITY;
+    }
+
+    const int64_t nstages = Npad / CB;
+    const bool cn_lane = threadIdx.x < CB / 4;
+    uint4 stage[NV];
+    float4 stage_cn = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(Xh + (int64_t)(v * 256 + threadIdx.x) * 16);
+    if (cn_lane) stage_cn = *(const float4 *)(cn + 4 * threadIdx.x);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        int u = v * 256 + threadIdx.x;
+        int r = u / CHUNKS, c = u - r * CHUNKS;
+        *(uint4 *)(&lds[0][r * LROWB + c * 16]) = stage[v];
+    }
+    if (cn_lane) *(float4 *)(&ldc[0][4 * threadIdx.x]) = stage_cn;
+    if (threadIdx.x < 32) ldinf[threadIdx.x] = INFINITY;
+    __syncthreads();
+
+    // set 1's accumulator of the previous block; before the first block it scores +inf
+    f32x16_t acc1 = f32x16_t{};
+    const float *pc1 = ldinf;
+    int64_t pc0 = 0;
+    int cb = 0;  // st % 3
+    for (int64_t st = 0; st < nstages; ++st) {
+        const int buf = (int)(st & 1);
+        {
+            const int64_t nx = (st + 1 < nstages) ? st + 1 : st;
+            const unsigned char *nb = Xh + nx * CB * (int64_t)ROWB;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(nb + (int64_t)(v * 256 + threadIdx.x) * 16);
+            if (cn_lane) stage_cn = *(const float4 *)(cn + nx * CB + 4 * threadIdx.x);
+        }
+#pragma unroll
+        for (int rb = 0; rb < NB; ++rb) {
+            const int64_t c0 = st * CB + rb * 32;
+            const unsigned char *arow = &lds[buf][(rb * 32 + col) * LROWB];
+            uint4 a[KS];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) a[s] = *(const uint4 *)(arow + s * 32 + h * 16);
+            f32x16_t acc0 = f32x16_t{};
+#pragma unroll
+            for (int s = 0; s < KS; ++s) acc0 = mfma_step<P>(a[s], qf[0][s], acc0);
+            const float m1 = block_scores(acc1, pc1, h, beta);
+            if (__any(m1 < thr[1])) block_insert<KL>(ld[1], li[1], thr[1], acc1, pc0, h);
+            acc1 = f32x16_t{};
+#pragma unroll
+            for (int s = 0; s < KS; ++s) acc1 = mfma_step<P>(a[s], qf[1][s], acc1);
+            const float *pc = &ldc[cb][rb * 32];
+            const float m0 = block_scores(acc0, pc, h, beta);
+            if (__any(m0 < thr[0])) block_insert<KL>(ld[0], li[0], thr[0], acc0, c0, h);
+            pc1 = pc;
+            pc0 = c0;
+        }
+        const int cbn = (cb == 2) ? 0 : cb + 1;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            int u = v * 256 + threadIdx.x;
+            int r = u / CHUNKS, c = u - r * CHUNKS;
+            *(uint4 *)(&lds[buf ^ 1][r * LROWB + c * 16]) = stage[v];
+        }
+        if (cn_lane) *(float4 *)(&ldc[cbn][4 * threadIdx.x]) = stage_cn;
+        cb = cbn;
+        __syncthreads();
+    }
+    {
+        const float m1 = block_scores(acc1, pc1, h, beta);
+        if (__any(m1 < thr[1])) block_insert<KL>(ld[1], li[1], thr[1], acc1, pc0, h);
     }
 
 #pragma unroll
@@ -845,6 +1016,28 @@ static void launch_tiles_dk(int DK, dim3 grid, const void *Xh, const float *cn, 
             default: throw HipError("[ncvis_amd] unsupported padded dimension");
         }
         return;
+    }
+    if constexpr (P != PREC_FP32 && QB == 2 && KL == 16) {
+        if (tu.knn_skew != 0) {
+            if (tu.knn_stage_mult == 2) {
+                switch (DK) {
+                    case 1: k_knn_tiles_sk<P, KL, 1, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+                    case 2: k_knn_tiles_sk<P, KL, 2, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+                    case 4: k_knn_tiles_sk<P, KL, 4, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+                    case 8: k_knn_tiles_sk<P, KL, 8, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+                    default: throw HipError("[ncvis_amd] unsupported padded dimension");
+                }
+            } else {
+                switch (DK) {
+                    case 1: k_knn_tiles_sk<P, KL, 1, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+                    case 2: k_knn_tiles_sk<P, KL, 2, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+                    case 4: k_knn_tiles_sk<P, KL, 4, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+                    case 8: k_knn_tiles_sk<P, KL, 8, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+                    default: throw HipError("[ncvis_amd] unsupported padded dimension");
+                }
+            }
+            return;
+        }
     }
     if (tu.knn_stage_mult == 2) {
         switch (DK) {

@@ -32,9 +32,9 @@ void ncvis::NCVis::set_option(const std::string &key, double value) {
     if (rc != 0) throw std::runtime_error(ncvis_last_error(h_));
 }
 
-void ncvis::NCVis::stats(double *stage_ms7, long long *n_edges, float *final_Q, long long *chains) const {
+void ncvis::NCVis::stats(double *stage_ms8, long long *n_edges, float *final_Q, long long *chains) const {
     int64_t e = 0, c = 0;
-    ncvis_get_stats(h_, stage_ms7, &e, final_Q, &c);
+    ncvis_get_stats(h_, stage_ms8, &e, final_Q, &c);
     if (n_edges) *n_edges = (long long)e;
     if (chains) *chains = (long long)c;
 }

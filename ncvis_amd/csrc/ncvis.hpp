@@ -46,8 +46,8 @@ class NCVis {
     void set_option(const std::string &key, double value);
 
     /*! @brief Stage times of the last fit in ms: knn, symmetrize, init, optimize, total, knn tiles,
-        knn rerank; plus edge count, final Q and the chain count used. */
-    void stats(double *stage_ms7, long long *n_edges, float *final_Q, long long *chains) const;
+        knn rerank, SGD epoch kernels; plus edge count, final Q and the chain count used. */
+    void stats(double *stage_ms8, long long *n_edges, float *final_Q, long long *chains) const;
 
    private:
     ncvis_handle *h_;

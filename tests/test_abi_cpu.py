@@ -78,3 +78,20 @@ def test_default_negative_plan():
     assert list(plan) == [0, 2, 3] and ne == 3
     plan, ne = negative_plan_for(4, 7)
     assert list(plan) == [4] * 7 and ne == 7
+
+
+def test_one_hip_runtime_whatever_the_import_order():
+    # the library must bind to the HIP runtime torch bundles, whether it is loaded before or
+    # after torch: two runtimes in one process compete for the GPU
+    import subprocess
+    import sys
+    pytest.importorskip("torch")
+    code = ("import re, sys; sys.path.insert(0, %r)\n"
+            "from ncvis_amd import abi, synthetic\n"
+            "abi.lib()\n"
+            "import torch\n"
+            "m = open('/proc/self/maps').read()\n"
+            "print(len(set(re.findall(r'\\S*libamdhip64\\S*', m))))\n") % ROOT
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip().splitlines()[-1] == "1"

@@ -133,6 +133,21 @@ def test_knn_exact_on_integer_data(torch_cuda, precision, shape):
     assert np.array_equal(gd, od)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+def test_knn_exact_with_duplicate_rows(torch_cuda, precision):
+    # 40 distinct integer rows repeated ~75 times each: every query has dozens of neighbours at
+    # distance 0, so the table is decided by the label tie-break alone (smaller label first,
+    # as the (distance, label) max-heap of the reference keeps them)
+    _say(f"knn duplicates {precision}")
+    rng = np.random.default_rng(11)
+    protos = rng.integers(0, 8, (40, 32)).astype(np.float32)
+    X = np.ascontiguousarray(protos[rng.integers(0, 40, 3000)])
+    gi, gd = gpu_knn(torch_cuda, X, 15, "euclidean", precision)
+    oi, od = oracle.knn_bruteforce(oracle.preprocess(X), 15)
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(gd, od)
+
+
 @pytest.mark.parametrize("shape", [(300, 300, 15), (200, 20, 40), (5, 3, 4)])
 def test_knn_generic_path(torch_cuda, shape):
     N, D, k = shape
