@@ -870,29 +870,66 @@ __device__ inline bool key_less(float da, int ia, int la, float db, int ib, int 
     return da < db || (da == db && (ia < ib || (ia == ib && la < lb)));
 }
 
-// One wave per query; lane c reranks candidate c (C = 2*KL <= 64 candidates).
+// Exact distance with row loads four floats at a time; the sum stays sequential and uncontracted
+// (same value as exact_dist). Rows must be 16-byte aligned (D % 4 == 0 and an aligned base).
+__device__ inline float exact_dist4(const float *__restrict__ x, const float *__restrict__ y, int D, int is_l2) {
+    const float4 *x4 = (const float4 *)x;
+    const float4 *y4 = (const float4 *)y;
+    float s = 0.f;
+    if (is_l2) {
+        for (int j = 0; j < D / 4; ++j) {
+            const float4 a = x4[j], b = y4[j];
+            float t = __fsub_rn(a.x, b.x);
+            s = __fadd_rn(s, __fmul_rn(t, t));
+            t = __fsub_rn(a.y, b.y);
+            s = __fadd_rn(s, __fmul_rn(t, t));
+            t = __fsub_rn(a.z, b.z);
+            s = __fadd_rn(s, __fmul_rn(t, t));
+            t = __fsub_rn(a.w, b.w);
+            s = __fadd_rn(s, __fmul_rn(t, t));
+        }
+        return s;
+    }
+    for (int j = 0; j < D / 4; ++j) {
+        const float4 a = x4[j], b = y4[j];
+        s = __fadd_rn(s, __fmul_rn(a.x, b.x));
+        s = __fadd_rn(s, __fmul_rn(a.y, b.y));
+        s = __fadd_rn(s, __fmul_rn(a.z, b.z));
+        s = __fadd_rn(s, __fmul_rn(a.w, b.w));
+    }
+    return __fsub_rn(1.0f, s);
+}
+
+// C = 2*KL candidates per query (32 or 64); a wave reranks 64/C queries, lane c of a query's
+// segment handles candidate c: exact FP32 distance, rank by (distance, label, slot) with
+// shuffles inside the segment, drop rank 0, write ranks k..1 (farthest first).
+template <int C>
 __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ Xp, int64_t N, int D, int is_l2,
-                                                    const int32_t *__restrict__ cand, int C, int k,
+                                                    int vec4, const int32_t *__restrict__ cand, int k,
                                                     int32_t *__restrict__ inds, float *__restrict__ dists,
                                                     int *__restrict__ err) {
+    constexpr int G = 64 / C;
     const int lane = threadIdx.x & 63;
-    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (q >= N) return;
-    int ci = (lane < C) ? cand[q * C + lane] : -1;
+    const int c = lane % C;
+    const int64_t q = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * G + lane / C;
+    const bool valid = q < N;
+    int ci = valid ? cand[q * C + c] : -1;
     float dv = INFINITY;
     if (ci >= 0) {
-        dv = exact_dist(Xp + q * D, Xp + (int64_t)ci * D, D, is_l2);
+        dv = vec4 ? exact_dist4(Xp + q * D, Xp + (int64_t)ci * D, D, is_l2)
+                  : exact_dist(Xp + q * D, Xp + (int64_t)ci * D, D, is_l2);
         if (dv != dv) dv = INFINITY;  // NaN input: never ranks ahead of a real candidate
     } else {
         ci = INT_MAX;
     }
     int rank = 0;
-    for (int j = 0; j < 64; ++j) {
-        float dj = __shfl(dv, j);
-        int ij = __shfl(ci, j);
-        rank += key_less(dj, ij, j, dv, ci, lane) ? 1 : 0;
+#pragma unroll 8
+    for (int j = 0; j < C; ++j) {
+        const float dj = __shfl(dv, j, C);
+        const int ij = __shfl(ci, j, C);
+        rank += key_less(dj, ij, j, dv, ci, c) ? 1 : 0;
     }
-    if (rank >= 1 && rank <= k) {
+    if (valid && rank >= 1 && rank <= k) {
         if (ci == INT_MAX) atomicExch(err, 1);
         inds[q * k + (k - rank)] = (ci == INT_MAX) ? 0 : ci;
         dists[q * k + (k - rank)] = dv;
@@ -1127,7 +1164,11 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
         }
         NCVIS_LAUNCH_CHECK();
         NCVIS_HIP_CHECK(hipEventRecord(e1, st));
-        k_knn_rerank<<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, cand.p, 2 * KL, k, d_inds, d_dists, err.p);
+        const int vec4 = (D % 4 == 0) && ((uintptr_t)Xp % 16 == 0);
+        if (KL == 16)
+            k_knn_rerank<32><<<cdiv(N, 8), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand.p, k, d_inds, d_dists, err.p);
+        else
+            k_knn_rerank<64><<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand.p, k, d_inds, d_dists, err.p);
         NCVIS_LAUNCH_CHECK();
         NCVIS_HIP_CHECK(hipEventRecord(e2, st));
     } else {

@@ -9,6 +9,8 @@
  *   src/ncvis.cpp:32-70    preprocess                      -> orc_preprocess
  *   src/ncvis.cpp:128-153  findKNN row order semantics     -> orc_knn_bruteforce (exact search
  *                          stands in for hnswlib, whose source is absent)
+ *   src/ncvis.cpp:72-153   buildKNN + findKNN with HNSW    -> orc_hnsw_knn (oracle/hnsw.c, the
+ *                          published HNSW algorithm; the CPU baseline's kNN stage)
  *   src/knntable.cpp:24-99 KNNTable::symmetrize            -> orc_symmetrize
  *   src/ncvis.cpp:155-173  build_edges                     -> CSR produced by orc_symmetrize
  *   src/ncvis.cpp:175-182  d_sqr                           -> d_sqr

@@ -76,6 +76,9 @@ def lib():
                                         C.c_float, C.c_float, _u64, C.c_int, _f32, _f64,
                                         C.POINTER(C.c_float)]
         L.orc_fit_transform.restype = C.c_int
+        L.orc_hnsw_knn.argtypes = [_f32, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_uint64, C.c_int, _i64, _f32, _f64]
+        L.orc_hnsw_knn.restype = C.c_long
         _lib = L
     return _lib
 
@@ -150,6 +153,20 @@ def knn_bruteforce_queries(Xp: np.ndarray, k: int, queries, distance: str = "euc
     lib().orc_knn_bruteforce_queries(Xp, N, D, k, DIST_CODES[distance], n_threads, q, len(q), inds,
                                      dists)
     return inds, dists
+
+
+def hnsw_knn(Xp: np.ndarray, k: int, distance: str = "euclidean", M: int = 8, ef_construction: int = 100,
+             seed: int = 42, n_threads: int = 0):
+    """HNSW kNN in the reference's row format (hnswlib's role in buildKNN/findKNN), plus
+    (build_seconds, search_seconds)."""
+    Xp = np.ascontiguousarray(Xp, np.float32)
+    N, D = Xp.shape
+    inds = np.empty((N, k), np.int64)
+    dists = np.empty((N, k), np.float32)
+    times = np.zeros(2, np.float64)
+    short = lib().orc_hnsw_knn(Xp, N, D, DIST_CODES[distance], M, ef_construction, k, seed, n_threads,
+                               inds, dists, times)
+    return inds, dists, (float(times[0]), float(times[1])), int(short)
 
 
 def knn_fp64(Xp: np.ndarray, k: int, distance: str = "euclidean", queries=None, block: int = 2048):

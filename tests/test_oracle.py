@@ -85,6 +85,40 @@ def test_knn_bruteforce_vs_fp64(dist):
     assert np.all(np.diff(d32, axis=1) <= 0)
 
 
+@pytest.mark.parametrize("dist", ["euclidean", "cosine"])
+def test_hnsw_knn_rows_and_recall(dist):
+    # the CPU HNSW (hnswlib's role; hnswlib itself is absent): every returned distance is the exact
+    # distance of its pair, rows are farthest first, the k+1-th result dropped is the row itself,
+    # and recall against exact search is high on clustered data at the Python defaults
+    if dist == "euclidean":
+        X, _ = synthetic.gaussian_mixture(3000, 16, 8, seed=4)
+    else:
+        X, _ = synthetic.normalized_mixture(N=3000, D=24, n_clusters=8, seed=4)
+    Xp = oracle.preprocess(X, dist)
+    hi, hd, times, short = oracle.hnsw_knn(Xp, 15, dist, M=8, ef_construction=100, seed=42)
+    assert short == 0 and times[0] > 0 and times[1] > 0
+    X64 = Xp.astype(np.float64)
+    rows = np.arange(3000)[:, None]
+    if dist == "euclidean":
+        exact = ((X64[rows] - X64[hi]) ** 2).sum(-1)
+    else:
+        exact = 1.0 - (X64[rows] * X64[hi]).sum(-1)
+    np.testing.assert_allclose(hd, exact, rtol=1e-4, atol=1e-5)
+    assert np.all(np.diff(hd, axis=1) <= 0)
+    assert all(len(set(r)) == 15 for r in hi)
+    assert np.mean([i not in set(r) for i, r in enumerate(hi)]) > 0.99
+    ei, _ = oracle.knn_bruteforce(Xp, 15, dist)
+    recall = np.mean([len(set(a) & set(b)) / 15 for a, b in zip(hi, ei)])
+    assert recall > 0.9, recall
+
+
+def test_hnsw_knn_deterministic_single_thread():
+    X, _ = synthetic.gaussian_mixture(1500, 12, 5, seed=6)
+    a = oracle.hnsw_knn(X, 10, "euclidean", seed=7, n_threads=1)
+    b = oracle.hnsw_knn(X, 10, "euclidean", seed=7, n_threads=1)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
 def _sym_fixture(golden_dir, name):
     z = np.load(os.path.join(golden_dir, name))
     return {k: z[k] for k in z.files}
