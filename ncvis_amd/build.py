@@ -4,6 +4,7 @@
   for gfx950 only.
 * ``_ncvis*.so``: the Cython extension (``_ncvis.pyx``) holding the Python API and the
   ``ncvis::NCVis`` C++ class (``csrc/ncvis.cpp``), linked against ``libncvis_amd.so``.
+* ``ncvis_cli``: the command-line harness (``csrc/ncvis_main.cpp``) on the same class.
 
 Usage: ``python -m ncvis_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -83,9 +84,23 @@ def build_cython(verbose: bool = False) -> str:
     return out
 
 
+def build_cli(verbose: bool = False) -> str:
+    """``ncvis_cli``: the reference's command-line harness (src/main.cpp) on the C++ class."""
+    out = os.path.join(PKG, "ncvis_cli")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++14", "-I", INCLUDE, "-I", CSRC,
+          os.path.join(CSRC, "ncvis_main.cpp"), os.path.join(CSRC, "ncvis.cpp"), "-o", tmp,
+          "-L", PKG, "-lncvis_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build(verbose: bool = False) -> None:
     build_hip(verbose)
     build_cython(verbose)
+    build_cli(verbose)
 
 
 if __name__ == "__main__":
