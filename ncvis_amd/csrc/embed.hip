@@ -65,6 +65,42 @@ __global__ void k_spmm_rows(const int64_t *__restrict__ row_ptr, const int32_t *
     }
 }
 
+// Same sums for d <= 4 with the dimension loop inside the edge loop: each edge's column index
+// and row are read once, four edges are fetched ahead of their (in-order) accumulation. Every
+// dimension's sum keeps the sequential edge order, so the result equals k_spmm_rows bit for bit.
+template <int DIM>
+__global__ void k_spmm_rows_d(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col, int64_t N,
+                              float alpha, const float *__restrict__ Yo, float *__restrict__ Yn) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int64_t b = row_ptr[i], e = row_ptr[i + 1];
+    float acc[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) acc[c] = 0.f;
+    int64_t p = b;
+    for (; p + 4 <= e; p += 4) {
+        int32_t j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = col[p + u];
+        float y[4][DIM];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) y[u][c] = Yo[(int64_t)j[u] * DIM + c];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) acc[c] = __fadd_rn(acc[c], __fmul_rn(alpha, y[u][c]));
+    }
+    for (; p < e; ++p) {
+        const int64_t j = col[p];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) acc[c] = __fadd_rn(acc[c], __fmul_rn(alpha, Yo[j * DIM + c]));
+    }
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) Yn[i * DIM + c] = acc[c];
+}
+
 // per-block FP64 column sums of Y (mode 0) or of (Y - mean)^2 (mode 1)
 template <int MODE>
 __global__ __launch_bounds__(256) void k_colstat(const float *__restrict__ Y, int64_t N, int d,
@@ -126,7 +162,13 @@ void init_embedding_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_
     for (int ep = 0; ep < n_init_epochs; ++ep) {
         const float *Yo = Ys[ep % 2];
         float *Yn = Ys[(ep + 1) % 2];
-        k_spmm_rows<<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, d_col, N, d, alpha, Yo, Yn);
+        switch (d) {
+            case 1: k_spmm_rows_d<1><<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, d_col, N, alpha, Yo, Yn); break;
+            case 2: k_spmm_rows_d<2><<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, d_col, N, alpha, Yo, Yn); break;
+            case 3: k_spmm_rows_d<3><<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, d_col, N, alpha, Yo, Yn); break;
+            case 4: k_spmm_rows_d<4><<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, d_col, N, alpha, Yo, Yn); break;
+            default: k_spmm_rows<<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, d_col, N, d, alpha, Yo, Yn); break;
+        }
         NCVIS_LAUNCH_CHECK();
         k_colstat<0><<<nb, 256, 0, st>>>(Yn, N, d, nullptr, partial.p);
         k_finalize<0><<<1, 1024, 0, st>>>(partial.p, nb, d, N, stats.p);

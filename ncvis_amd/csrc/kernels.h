@@ -12,7 +12,7 @@ struct Tuning {
     int sgd_plain_far = 0;           // 1: far-endpoint updates with plain stores instead of atomics
     int sgd_fast_math = 0;           // 1: single-precision powf/expf in the many-chain epoch
     int64_t edges_per_chain = 64;    // auto chain count = E / edges_per_chain ...
-    int64_t max_chains = 1 << 18;    // ... capped here
+    int64_t max_chains = 1 << 17;    // ... capped here
     int knn_dma = -1;                // kNN tiles fetch stages with LDS DMA (1), via registers (0), auto (-1)
     int knn_select = -1;             // kNN top-k: register lists (0), buffered in HBM (1), auto (-1)
     int knn_stage_mult = 2;          // kNN LDS stage rows x1 or x2 (register-staged kernels)
