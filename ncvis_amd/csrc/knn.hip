@@ -696,6 +696,12 @@ __device__ __forceinline__ void dma_stage(const unsigned char *__restrict__ Xh, 
 // s_waitcnt with only the vector-memory counter constrained (gfx9 encoding)
 #define NCVIS_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
 
+// Workgroup barrier without a memory fence: the stage pipelines order their LDS transfers with
+// explicit vector-memory waits, and a fenced barrier would also wait for the transfers of the
+// stages still in flight. The asm "memory" clobber keeps the compiler from moving LDS accesses
+// across it.
+#define NCVIS_BARRIER_NOFENCE() asm volatile("s_barrier" ::: "memory")
+
 template <int P, int QB, int KL, int DK>
 __global__ __launch_bounds__(256) void k_knn_tiles_dma(const void *__restrict__ Xh_, const float *__restrict__ cn,
                                                        int64_t N, int64_t Npad, float beta,
@@ -822,7 +828,7 @@ __global__ __launch_bounds__(256) void k_knn_tiles_dma(const void *__restrict__ 
 #define NCVIS_DMA_STEP(ST, LB, LC, NLB, NLC)                                                           \
     do {                                                                                               \
         NCVIS_WAIT_VMCNT(NV + 1);                                                                      \
-        __syncthreads();                                                                               \
+        NCVIS_BARRIER_NOFENCE();                                                                       \
         NCVIS_DMA((ST) + 2, NLB, NLC);                                                                 \
         NCVIS_DMA_COMPUTE(LB, LC, ST);                                                                 \
     } while (0)
