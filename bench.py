@@ -13,7 +13,10 @@ restatement on this host's cores with an HNSW kNN stage at the reference's Pytho
 recall@15 on the same queries is reported too), init and SGD timed on a bounded sample and scaled
 to the whole fit.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1m|digits|100k]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1m|digits|100k|batch64]
+
+`--workload batch64` times 64 small data sets through ncvis_fit_transform_many against the same
+fits one call at a time (a separate line; the headline is the 1m workload).
 """
 from __future__ import annotations
 
@@ -116,12 +119,65 @@ def cpu_baseline(X, plan, a, b, k, n_init_epochs, seed, distance="euclidean"):
     return out, h_inds
 
 
+def run_batch(args):
+    """64 independent data sets (N log-uniform in [2,000, 50,000], D in {32, 64, 128}, 5-30
+    clusters each): one ncvis_fit_transform_many call vs the same fits one by one."""
+    import torch
+    from ncvis_amd import abi, synthetic
+    from ncvis_amd._ncvis import find_ab_params, negative_plan_for
+
+    rng = np.random.default_rng(5)
+    shapes = [(int(np.exp(rng.uniform(np.log(2000), np.log(50000)))), int(rng.choice([32, 64, 128])),
+               int(rng.integers(5, 31))) for _ in range(64)]
+    sets = [synthetic.gaussian_mixture(n, dd, c, seed=100 + i)[0] for i, (n, dd, c) in enumerate(shapes)]
+    a, b = find_ab_params(1.0, 0.4)
+    plan, n_epochs = negative_plan_for(None, 50)
+    h = abi.Handle(d=2, n_threads=0, n_neighbors=15, random_seed=42, n_init_epochs=20, a=a, b=b,
+                   n_noise=list(plan), distance="euclidean")
+    h.set_option("knn_precision", abi.PREC[args.precision])
+    dX = [torch.from_numpy(X).cuda() for X in sets]
+    dY = [torch.empty((x.shape[0], 2), dtype=torch.float32, device="cuda") for x in dX]
+    jobs = [(x.data_ptr(), x.shape[0], x.shape[1], y.data_ptr()) for x, y in zip(dX, dY)]
+
+    def one_by_one():
+        for x, y in zip(dX, dY):
+            h.fit_transform_device(x.data_ptr(), x.shape[0], x.shape[1], y.data_ptr())
+
+    def batched():
+        h.fit_transform_many(jobs, n_streams=args.streams)
+
+    res = {}
+    for name, fn in (("one_by_one", one_by_one), ("batched", batched)):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / args.steps
+        log(f"{name}: {res[name]:.3f} s")
+    assert all(bool(torch.isfinite(y).all()) for y in dY), "non-finite embedding"
+    out = {
+        "metric": "fit_transform_many sec @64 data sets (N 2k-50k, D 32-128)",
+        "value": round(res["batched"], 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(res["batched"] * 1e3, 2), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": f"{args.precision}-knn/f32", "data": "synthetic",
+        "config": {"workload": "64 Gaussian mixtures, N log-uniform [2000, 50000], D in {32,64,128}",
+                   "total_points": int(sum(n for n, _, _ in shapes)), "streams": args.streams},
+        "one_by_one_s": round(res["one_by_one"], 4),
+        "speedup_batched": round(res["one_by_one"] / res["batched"], 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="1m", choices=sorted(WORKLOADS) + ["batch64"])
+    ap.add_argument("--streams", type=int, default=8, help="streams of the batch64 workload")
     ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
@@ -135,6 +191,9 @@ def main():
     from ncvis_amd import abi
     from ncvis_amd._ncvis import find_ab_params, negative_plan_for
 
+    if args.workload == "batch64":
+        run_batch(args)
+        return
     w = WORKLOADS[args.workload]
     N, D, k, d = w["N"], w["D"], 15, 2
     seed = 42

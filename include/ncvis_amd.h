@@ -73,6 +73,16 @@ int ncvis_fit_transform(ncvis_handle *h, const float *X, size_t N, size_t D, flo
 /* Same with X and Y already in device memory (HBM). */
 int ncvis_fit_transform_device(ncvis_handle *h, const float *dX, size_t N, size_t D, float *dY, void *stream);
 
+/* Many independent data sets in one call, each fitted with this handle's parameters exactly as
+ * ncvis_fit_transform_device fits it (the reference's fit_transform, src/ncvis.cpp:354-459, takes
+ * one data set; this batched form has no counterpart there). dX[i] [N[i], D[i]] and dY[i] [N[i], d]
+ * are device pointers. Data sets run n_streams at a time, each on a stream of its own driven by
+ * a host worker thread, so small data sets fill the GPU together. Blocking: waits for `stream`
+ * (where the inputs were produced) first and returns when every fit is done. An error names the
+ * first failing data set; the handle's stage times are left unchanged. */
+int ncvis_fit_transform_many(ncvis_handle *h, size_t count, const float *const *dX, const size_t *N, const size_t *D,
+                             float *const *dY, int n_streams, void *stream);
+
 /* Stage times of the last fit (milliseconds): [knn, symmetrize, init, optimize, total, knn_tiles,
  * knn_rerank, sgd_kernel] (sgd_kernel: the epoch kernels alone, summed over epochs), the edge
  * count E, the final Q and the effective chain count. */

@@ -16,7 +16,7 @@ EXPORTS = (
     "ncvis_create", "ncvis_destroy", "ncvis_last_error", "ncvis_set_option", "ncvis_fit_transform",
     "ncvis_fit_transform_device", "ncvis_get_stats", "ncvis_get_last_error", "ncvis_knn_device",
     "ncvis_symmetrize_device", "ncvis_init_embedding_device", "ncvis_optimize_device",
-    "ncvis_nce_loss_device", "ncvis_tune",
+    "ncvis_nce_loss_device", "ncvis_tune", "ncvis_fit_transform_many",
 )
 
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
@@ -45,6 +45,8 @@ def lib():
         L.ncvis_set_option.argtypes = [vp, C.c_char_p, C.c_double]
         L.ncvis_fit_transform.argtypes = [vp, vp, C.c_size_t, C.c_size_t, vp]
         L.ncvis_fit_transform_device.argtypes = [vp, vp, C.c_size_t, C.c_size_t, vp, vp]
+        L.ncvis_fit_transform_many.argtypes = [vp, C.c_size_t, C.POINTER(vp), C.POINTER(C.c_size_t),
+                                               C.POINTER(C.c_size_t), C.POINTER(vp), C.c_int, vp]
         L.ncvis_get_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                       C.POINTER(C.c_float), C.POINTER(C.c_int64)]
         L.ncvis_knn_device.argtypes = [vp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int, C.c_int, vp,
@@ -148,6 +150,16 @@ class Handle:
 
     def fit_transform_device(self, dX: int, N: int, D: int, dY: int, stream=None) -> None:
         self._hcheck(lib().ncvis_fit_transform_device(self._h, dX, N, D, dY, _stream(stream)))
+
+    def fit_transform_many(self, jobs, n_streams: int = 4, stream=None) -> None:
+        """jobs: sequence of (dX, N, D, dY) device pointers and sizes, fitted n_streams at a time."""
+        n = len(jobs)
+        px = (vp * n)(*[j[0] for j in jobs])
+        pn = (C.c_size_t * n)(*[j[1] for j in jobs])
+        pd = (C.c_size_t * n)(*[j[2] for j in jobs])
+        py = (vp * n)(*[j[3] for j in jobs])
+        self._hcheck(lib().ncvis_fit_transform_many(self._h, n, px, pn, pd, py, int(n_streams),
+                                                    _stream(stream)))
 
     def fit_transform_host(self, X, Y) -> None:
         self._hcheck(lib().ncvis_fit_transform(self._h, X.ctypes.data, X.shape[0], X.shape[1],

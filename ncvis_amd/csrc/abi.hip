@@ -2,8 +2,11 @@
 // and the device-pointer stage entry points. Declarations and reference citations are in
 // include/ncvis_amd.h.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -70,6 +73,82 @@ static int64_t resolve_chains(const ncvis_handle *h, int64_t E) {
     if (h->chains_override > 0) return h->chains_override;
     if (h->n_threads > 0) return (int64_t)h->n_threads;
     return auto_chains(E);
+}
+
+struct FitStats {
+    double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t n_edges = 0;
+    float Q = 0.f;
+    int64_t chains = 0;
+};
+
+// One whole fit (ncvis.cpp:354-459) on device buffers, queued on `st`; reads the handle's
+// parameters only, so fits of several data sets may run concurrently on their own streams.
+static void fit_device(const ncvis_handle *h, const float *dX, size_t N, size_t D, float *dY, hipStream_t st,
+                       FitStats &out) {
+    // ncvis.cpp:364-369
+    if (N == 0 || D == 0)
+        throw std::invalid_argument("[ncvis::NCVis::fit_transform] Dataset should have at least one element.");
+    if (dY == nullptr)
+        throw std::invalid_argument("[ncvis::NCVis::fit_transform] Null pointer provided for output.");
+    if (N < 2) throw std::invalid_argument("[ncvis::NCVis::fit_transform] At least two points are required.");
+    if (N > (size_t)INT32_MAX - 1024) throw std::invalid_argument("[ncvis_amd] more than 2^31 points");
+    // ncvis.cpp:381-382
+    size_t kk = (h->n_neighbors < N - 1) ? h->n_neighbors : (N - 1);
+    kk = (kk > 0) ? kk : 1;
+    const int k = (int)kk;
+    const int64_t n = (int64_t)N;
+    hipEvent_t ev[5];
+    for (auto &e : ev) NCVIS_HIP_CHECK(hipEventCreate(&e));
+    struct EvGuard {
+        hipEvent_t *e;
+        ~EvGuard() {
+            for (int i = 0; i < 5; ++i) (void)hipEventDestroy(e[i]);
+        }
+    } evg{ev};
+    NCVIS_HIP_CHECK(hipEventRecord(ev[0], st));
+    DevBuf<int32_t> inds((size_t)n * k);
+    DevBuf<float> dists((size_t)n * k);
+    KnnStats ks;
+    knn_run(dX, n, (int)D, k, h->dist, h->precision, inds.p, dists.p, st, &ks);
+    NCVIS_HIP_CHECK(hipEventRecord(ev[1], st));
+    DevBuf<int64_t> row_ptr((size_t)n + 1);
+    DevBuf<int32_t> col((size_t)2 * n * k);
+    DevBuf<float> cdist((size_t)2 * n * k);
+    const int64_t E = symmetrize_run(inds.p, dists.p, n, k, row_ptr.p, col.p, cdist.p, st);
+    inds.release();
+    dists.release();
+    cdist.release();
+    NCVIS_HIP_CHECK(hipEventRecord(ev[2], st));
+    const int64_t T = resolve_chains(h, E);
+    const float init_alpha = (float)(1. / k);  // ncvis.cpp:418
+    init_embedding_run(n, (int)h->d, row_ptr.p, col.p, init_alpha, h->n_init_epochs, h->random_seed, T, dY, st);
+    NCVIS_HIP_CHECK(hipEventRecord(ev[3], st));
+    SgdParams p;
+    p.a = h->a;
+    p.b = h->b;
+    p.alpha = h->alpha;
+    p.alpha_Q = h->alpha_Q;
+    p.seed = h->random_seed;
+    p.n_epochs = h->n_epochs;
+    p.n_noise = h->n_noise.data();
+    p.chains = T;
+    p.atomic_updates = h->atomic_updates >= 0 ? h->atomic_updates : (T > 1 ? 1 : 0);
+    float Q = 0.f;  // ncvis.cpp:417
+    double epoch_kernel_ms = 0.0;
+    optimize_run(n, (int)h->d, row_ptr.p, col.p, E, dY, &Q, p, st, nullptr, &epoch_kernel_ms);
+    NCVIS_HIP_CHECK(hipEventRecord(ev[4], st));
+    NCVIS_HIP_CHECK(hipEventSynchronize(ev[4]));
+    float ms[4];
+    for (int i = 0; i < 4; ++i) NCVIS_HIP_CHECK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+    for (int i = 0; i < 4; ++i) out.ms[i] = ms[i];
+    out.ms[4] = ms[0] + ms[1] + ms[2] + ms[3];
+    out.ms[5] = ks.tile_ms;
+    out.ms[6] = ks.rerank_ms;
+    out.ms[7] = epoch_kernel_ms;
+    out.n_edges = E;
+    out.Q = Q;
+    out.chains = T;
 }
 
 extern "C" {
@@ -179,70 +258,64 @@ int ncvis_tune(const char *key, double value) {
 int ncvis_fit_transform_device(ncvis_handle *h, const float *dX, size_t N, size_t D, float *dY, void *stream) {
     if (!h) return NCVIS_EINVAL;
     return guarded(h->last_error, [&] {
-        // ncvis.cpp:364-369
-        if (N == 0 || D == 0)
-            throw std::invalid_argument("[ncvis::NCVis::fit_transform] Dataset should have at least one element.");
-        if (dY == nullptr)
-            throw std::invalid_argument("[ncvis::NCVis::fit_transform] Null pointer provided for output.");
-        if (N < 2) throw std::invalid_argument("[ncvis::NCVis::fit_transform] At least two points are required.");
-        if (N > (size_t)INT32_MAX - 1024) throw std::invalid_argument("[ncvis_amd] more than 2^31 points");
-        hipStream_t st = stream ? (hipStream_t)stream : h->stream;
-        // ncvis.cpp:381-382
-        size_t kk = (h->n_neighbors < N - 1) ? h->n_neighbors : (N - 1);
-        kk = (kk > 0) ? kk : 1;
-        const int k = (int)kk;
-        const int64_t n = (int64_t)N;
-        hipEvent_t ev[5];
-        for (auto &e : ev) NCVIS_HIP_CHECK(hipEventCreate(&e));
-        struct EvGuard {
-            hipEvent_t *e;
-            ~EvGuard() {
-                for (int i = 0; i < 5; ++i) (void)hipEventDestroy(e[i]);
+        FitStats fs;
+        fit_device(h, dX, N, D, dY, stream ? (hipStream_t)stream : h->stream, fs);
+        for (int i = 0; i < 8; ++i) h->stage_ms[i] = fs.ms[i];
+        h->n_edges = fs.n_edges;
+        h->final_Q = fs.Q;
+        h->chains_used = fs.chains;
+    });
+}
+
+int ncvis_fit_transform_many(ncvis_handle *h, size_t count, const float *const *dX, const size_t *N, const size_t *D,
+                             float *const *dY, int n_streams, void *stream) {
+    if (!h) return NCVIS_EINVAL;
+    return guarded(h->last_error, [&] {
+        if (count == 0) return;
+        if (!dX || !N || !D || !dY) throw std::invalid_argument("[ncvis_amd] fit_transform_many: null array");
+        // inputs may still be in flight on the caller's stream
+        NCVIS_HIP_CHECK(hipStreamSynchronize(stream ? (hipStream_t)stream : h->stream));
+        const int S = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(n_streams, 1), count));
+        std::atomic<size_t> next{0};
+        std::mutex mu;
+        std::string first_error;
+        size_t first_index = count;
+        auto worker = [&]() {
+            hipStream_t ws = nullptr;
+            std::string err;
+            int rc = guarded(err, [&] { NCVIS_HIP_CHECK(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking)); });
+            if (rc == NCVIS_OK) {
+                // stream-ordered allocations: hipFree would wait for every stream of the device
+                alloc_stream() = ws;
+                for (size_t i = next++; i < count; i = next++) {
+                    FitStats fs;
+                    rc = guarded(err, [&] { fit_device(h, dX[i], N[i], D[i], dY[i], ws, fs); });
+                    if (rc != NCVIS_OK) {
+                        std::lock_guard<std::mutex> lk(mu);
+                        if (i < first_index) {
+                            first_index = i;
+                            first_error = err;
+                        }
+                    }
+                }
+                (void)hipStreamSynchronize(ws);
+                alloc_stream() = nullptr;
+                (void)hipStreamDestroy(ws);
+            } else {
+                std::lock_guard<std::mutex> lk(mu);
+                if (first_error.empty()) first_error = err;
             }
-        } evg{ev};
-        NCVIS_HIP_CHECK(hipEventRecord(ev[0], st));
-        DevBuf<int32_t> inds((size_t)n * k);
-        DevBuf<float> dists((size_t)n * k);
-        KnnStats ks;
-        knn_run(dX, n, (int)D, k, h->dist, h->precision, inds.p, dists.p, st, &ks);
-        NCVIS_HIP_CHECK(hipEventRecord(ev[1], st));
-        DevBuf<int64_t> row_ptr((size_t)n + 1);
-        DevBuf<int32_t> col((size_t)2 * n * k);
-        DevBuf<float> cdist((size_t)2 * n * k);
-        const int64_t E = symmetrize_run(inds.p, dists.p, n, k, row_ptr.p, col.p, cdist.p, st);
-        inds.release();
-        dists.release();
-        cdist.release();
-        NCVIS_HIP_CHECK(hipEventRecord(ev[2], st));
-        const int64_t T = resolve_chains(h, E);
-        const float init_alpha = (float)(1. / k);  // ncvis.cpp:418
-        init_embedding_run(n, (int)h->d, row_ptr.p, col.p, init_alpha, h->n_init_epochs, h->random_seed, T, dY, st);
-        NCVIS_HIP_CHECK(hipEventRecord(ev[3], st));
-        SgdParams p;
-        p.a = h->a;
-        p.b = h->b;
-        p.alpha = h->alpha;
-        p.alpha_Q = h->alpha_Q;
-        p.seed = h->random_seed;
-        p.n_epochs = h->n_epochs;
-        p.n_noise = h->n_noise.data();
-        p.chains = T;
-        p.atomic_updates = h->atomic_updates >= 0 ? h->atomic_updates : (T > 1 ? 1 : 0);
-        float Q = 0.f;  // ncvis.cpp:417
-        double epoch_kernel_ms = 0.0;
-        optimize_run(n, (int)h->d, row_ptr.p, col.p, E, dY, &Q, p, st, nullptr, &epoch_kernel_ms);
-        NCVIS_HIP_CHECK(hipEventRecord(ev[4], st));
-        NCVIS_HIP_CHECK(hipEventSynchronize(ev[4]));
-        float ms[4];
-        for (int i = 0; i < 4; ++i) NCVIS_HIP_CHECK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
-        for (int i = 0; i < 4; ++i) h->stage_ms[i] = ms[i];
-        h->stage_ms[4] = ms[0] + ms[1] + ms[2] + ms[3];
-        h->stage_ms[5] = ks.tile_ms;
-        h->stage_ms[6] = ks.rerank_ms;
-        h->stage_ms[7] = epoch_kernel_ms;
-        h->n_edges = E;
-        h->final_Q = Q;
-        h->chains_used = T;
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < S; ++t) pool.emplace_back(worker);
+        worker();
+        for (auto &t : pool) t.join();
+        if (!first_error.empty()) {
+            if (first_index < count)
+                throw std::runtime_error("[ncvis_amd] fit_transform_many: data set " + std::to_string(first_index) +
+                                         ": " + first_error);
+            throw std::runtime_error(first_error);
+        }
     });
 }
 

@@ -24,38 +24,61 @@ struct HipError : std::runtime_error {
 
 #define NCVIS_LAUNCH_CHECK() NCVIS_HIP_CHECK(hipGetLastError())
 
+// Stream for stream-ordered allocations made by this host thread (null: hipMalloc / hipFree).
+// Batched fits run one data set per worker thread and stream; hipFree would wait for every
+// stream of the device and serialise them.
+inline hipStream_t &alloc_stream() {
+    static thread_local hipStream_t s = nullptr;
+    return s;
+}
+
 // Device memory owned by a host object (freed on scope exit).
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    hipStream_t owner = nullptr;  // stream of a stream-ordered allocation
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) {
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n), owner(o.owner) {
         o.p = nullptr;
         o.n = 0;
+        o.owner = nullptr;
     }
     DevBuf &operator=(DevBuf &&o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             n = o.n;
+            owner = o.owner;
             o.p = nullptr;
             o.n = 0;
+            o.owner = nullptr;
         }
         return *this;
     }
     void alloc(size_t count) {
         release();
         n = count;
-        if (count) NCVIS_HIP_CHECK(hipMalloc((void **)&p, sizeof(T) * count));
+        if (!count) return;
+        owner = alloc_stream();
+        if (owner)
+            NCVIS_HIP_CHECK(hipMallocAsync((void **)&p, sizeof(T) * count, owner));
+        else
+            NCVIS_HIP_CHECK(hipMalloc((void **)&p, sizeof(T) * count));
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (owner)
+                (void)hipFreeAsync(p, owner);
+            else
+                (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
+        owner = nullptr;
     }
     ~DevBuf() { release(); }
 };
