@@ -62,7 +62,8 @@ int ncvis_set_option(ncvis_handle *h, const char *key, double value);
 
 /* Process-wide tuning knobs (no reference counterpart). None changes a single-chain fit; the kNN
  * knobs never change the kNN table. Keys: "sgd_prefetch" (1,2,4,8), "sgd_plain_far" (0/1),
- * "sgd_fast_math" (0/1), "edges_per_chain" (auto chain count = E / this), "max_chains",
+ * "sgd_fast_math" (0/1), "edges_per_chain" (auto chain count = E / this, at least
+ * min(E / 16, "min_chains"), at most "max_chains"),
  * "knn_dma" (-1 auto, 0, 1), "knn_select" (-1 auto, 0 register lists, 1 buffered),
  * "knn_stage_mult" (1, 2), "knn_skew" (0/1: skewed two-set 16-bit tile kernel). */
 int ncvis_tune(const char *key, double value);

@@ -63,10 +63,15 @@ static int guarded(std::string &err, F &&f) {
     }
 }
 
-// GPU-sized number of SGD chains: E / edges_per_chain, at most max_chains (tuning knobs).
+// GPU-sized chain count: E / edges_per_chain, within [min(E / 16, min_chains), max_chains].
+// min_chains is off by default: more, shorter chains make a small graph's epochs faster (their
+// time is the per-sample latency of a long chain) but move its result away from the reference's
+// few-thread Hogwild (the reference's own 1-D two-cluster test fails at E / 16 chains).
 static int64_t auto_chains(int64_t E) {
     const Tuning &t = tuning();
-    return std::max<int64_t>(1, std::min<int64_t>(E / std::max<int64_t>(1, t.edges_per_chain), t.max_chains));
+    int64_t c = E / std::max<int64_t>(1, t.edges_per_chain);
+    c = std::max<int64_t>(c, std::min<int64_t>(E / 16, t.min_chains));
+    return std::max<int64_t>(1, std::min<int64_t>(c, t.max_chains));
 }
 
 static int64_t resolve_chains(const ncvis_handle *h, int64_t E) {
@@ -246,6 +251,9 @@ int ncvis_tune(const char *key, double value) {
             t.knn_stage_mult = (int)value;
         } else if (k == "knn_select") {
             t.knn_select = value < 0 ? -1 : (value != 0);
+        } else if (k == "min_chains") {
+            if (value < 1) throw std::invalid_argument("[ncvis_amd] min_chains must be >= 1");
+            t.min_chains = (int64_t)value;
         } else if (k == "max_chains") {
             if (value < 1) throw std::invalid_argument("[ncvis_amd] max_chains must be >= 1");
             t.max_chains = (int64_t)value;

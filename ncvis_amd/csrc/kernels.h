@@ -11,8 +11,9 @@ struct Tuning {
     int sgd_prefetch = 1;            // rows fetched together per chain in the many-chain epoch (1,2,4,8)
     int sgd_plain_far = 0;           // 1: far-endpoint updates with plain stores instead of atomics
     int sgd_fast_math = 0;           // 1: single-precision powf/expf in the many-chain epoch
-    int64_t edges_per_chain = 64;    // auto chain count = E / edges_per_chain ...
-    int64_t max_chains = 1 << 17;    // ... capped here
+    int64_t edges_per_chain = 64;    // auto chain count = E / edges_per_chain,
+    int64_t min_chains = 1;          // ... at least min(E / 16, min_chains) (1: off) ...
+    int64_t max_chains = 1 << 17;    // ... and at most this
     int knn_dma = -1;                // kNN tiles fetch stages with LDS DMA (1), via registers (0), auto (-1)
     int knn_select = -1;             // kNN top-k: register lists (0), buffered in HBM (1), auto (-1)
     int knn_stage_mult = 2;          // kNN LDS stage rows x1 or x2 (register-staged kernels)
