@@ -1117,6 +1117,29 @@ static void launch_tiles(int KL, int DK, int64_t N, int64_t Npad, const void *Xh
         launch_tiles_dk<P, QB, 32>(DK, grid, Xh, cn, N, Npad, beta, cand, st);
 }
 
+// Rows of 257..512 elements (16-bit GEMM): one query set per wave, so the query fragments
+// (Dpad / 16 per lane) still fit in VGPRs; 32-row stages.
+template <int P, int KL>
+static void launch_tiles_wide_kl(int DK, dim3 grid, const void *Xh, const float *cn, int64_t N, int64_t Npad,
+                                 float beta, int32_t *cand, hipStream_t st) {
+    switch (DK) {
+        case 10: k_knn_tiles<P, 1, KL, 10><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+        case 12: k_knn_tiles<P, 1, KL, 12><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+        case 16: k_knn_tiles<P, 1, KL, 16><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+        default: throw HipError("[ncvis_amd] unsupported padded dimension");
+    }
+}
+
+template <int P>
+static void launch_tiles_wide(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
+                              int32_t *cand, hipStream_t st) {
+    dim3 grid((unsigned)(Npad / (4 * 32)));
+    if (KL == 16)
+        launch_tiles_wide_kl<P, 16>(DK, grid, Xh, cn, N, Npad, beta, cand, st);
+    else
+        launch_tiles_wide_kl<P, 32>(DK, grid, Xh, cn, N, Npad, beta, cand, st);
+}
+
 void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32_t *d_inds, float *d_dists,
              hipStream_t st, KnnStats *stats) {
     if (N < 2 || D < 1 || k < 1 || k > N - 1) throw HipError("[ncvis_amd] knn: invalid sizes");
@@ -1136,11 +1159,14 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
     NCVIS_HIP_CHECK(hipEventCreate(&e0));
     NCVIS_HIP_CHECK(hipEventCreate(&e1));
     NCVIS_HIP_CHECK(hipEventCreate(&e2));
-    const bool tiled = (D <= 256) && (k + 1 <= 32);
+    // 16-bit GEMMs tile rows up to 512 elements, FP32 up to 256; wider rows or k + 1 > 32 take
+    // the generic exact scan
+    const bool wide = (prec != PREC_FP32) && (D > 256) && (D <= 512);
+    const bool tiled = ((D <= 256) || wide) && (k + 1 <= 32);
     if (tiled) {
-        const int DK = D <= 32 ? 1 : D <= 64 ? 2 : D <= 128 ? 4 : 8;
+        const int DK = D <= 32 ? 1 : D <= 64 ? 2 : D <= 128 ? 4 : D <= 256 ? 8 : D <= 320 ? 10 : D <= 384 ? 12 : 16;
         const int Dpad = 32 * DK;
-        const int QB = (prec == PREC_FP32) ? qb_for<PREC_FP32>() : qb_for<PREC_BF16>();
+        const int QB = wide ? 1 : (prec == PREC_FP32) ? qb_for<PREC_FP32>() : qb_for<PREC_BF16>();
         const int64_t qtile = 4 * 32 * QB;
         const int64_t Npad = ((N + qtile - 1) / qtile) * qtile;
         const int esz = (prec == PREC_FP32) ? 4 : 2;
@@ -1163,10 +1189,17 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
         DevBuf<int32_t> cand((size_t)N * 2 * KL);
         const float beta = is_l2 ? -2.0f : -1.0f;
         NCVIS_HIP_CHECK(hipEventRecord(e0, st));
-        switch (prec) {
-            case PREC_FP32: launch_tiles<PREC_FP32>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
-            case PREC_BF16: launch_tiles<PREC_BF16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
-            default: launch_tiles<PREC_FP16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
+        if (wide) {
+            if (prec == PREC_BF16)
+                launch_tiles_wide<PREC_BF16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
+            else
+                launch_tiles_wide<PREC_FP16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
+        } else {
+            switch (prec) {
+                case PREC_FP32: launch_tiles<PREC_FP32>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
+                case PREC_BF16: launch_tiles<PREC_BF16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
+                default: launch_tiles<PREC_FP16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
+            }
         }
         NCVIS_LAUNCH_CHECK();
         NCVIS_HIP_CHECK(hipEventRecord(e1, st));

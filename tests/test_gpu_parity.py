@@ -120,7 +120,8 @@ def test_init_embedding_matches_oracle(torch_cuda, n_init):
 
 # ------------------------------------------------------------------------------ kNN
 @pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
-@pytest.mark.parametrize("shape", [(1500, 64, 15), (700, 32, 5), (333, 100, 31), (40, 16, 15)])
+@pytest.mark.parametrize("shape", [(1500, 64, 15), (700, 32, 5), (333, 100, 31), (40, 16, 15),
+                                   (600, 300, 15), (500, 257, 10), (300, 512, 31), (450, 384, 15)])
 def test_knn_exact_on_integer_data(torch_cuda, precision, shape):
     # integer data: every precision computes exact candidate scores, so the table equals the
     # exact (distance, label) order bit for bit, ties included
@@ -169,6 +170,20 @@ def test_knn_inner_product_family(torch_cuda, distance, precision):
     agree = np.mean([len(set(a) & set(b)) / 15 for a, b in zip(gi, oi)])
     assert agree > 0.995
     np.testing.assert_allclose(np.sort(gd, 1), np.sort(od, 1), atol=2e-5, rtol=1e-5)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_knn_recall_wide_rows_cosine(torch_cuda, precision):
+    # rows of 300 elements (the 100k x 300 cosine configuration's shape) take the one-set-per-wave
+    # tiles of the 16-bit GEMM
+    _say(f"knn wide cosine {precision}")
+    X, _ = synthetic.normalized_mixture(N=3000, D=300, n_clusters=12, seed=3)
+    gi, gd = gpu_knn(torch_cuda, X, 15, "cosine", precision)
+    Xp = oracle.preprocess(X, "cosine")
+    oi, od = oracle.knn_bruteforce(Xp, 15, "cosine")
+    recall = np.mean([len(set(a) & set(b)) / 15 for a, b in zip(gi, oi)])
+    assert recall >= 0.99, recall
+    assert np.all(np.diff(gd, axis=1) <= 0)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
