@@ -184,7 +184,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the fit path runs on a single MI355X in this build "
-              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+              "(sharding over several GPUs is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import torch

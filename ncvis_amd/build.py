@@ -27,7 +27,7 @@ HIP_SOURCES = ["knn.hip", "symmetrize.hip", "embed.hip", "abi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
-    # packed FP32 instructions stay off (see the build notes in DESIGN)
+    # packed FP32 instructions stay off: some of their forms have given wrong results on this part
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", INCLUDE, "-I", CSRC,
 ]
