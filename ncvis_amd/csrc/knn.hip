@@ -942,6 +942,49 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ Xp
     }
 }
 
+// 2*KL = 128 candidates per query (k + 1 up to 64): a wave per query, lanes l and l + 64 of the
+// candidate list on each lane; same ranking and output as k_knn_rerank.
+__global__ __launch_bounds__(256) void k_knn_rerank128(const float *__restrict__ Xp, int64_t N, int D, int is_l2,
+                                                       int vec4, const int32_t *__restrict__ cand, int k,
+                                                       int32_t *__restrict__ inds, float *__restrict__ dists,
+                                                       int *__restrict__ err) {
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const bool valid = q < N;
+    int ci[2];
+    float dv[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        ci[u] = valid ? cand[q * 128 + lane + 64 * u] : -1;
+        dv[u] = INFINITY;
+        if (ci[u] >= 0) {
+            dv[u] = vec4 ? exact_dist4(Xp + q * D, Xp + (int64_t)ci[u] * D, D, is_l2)
+                         : exact_dist(Xp + q * D, Xp + (int64_t)ci[u] * D, D, is_l2);
+            if (dv[u] != dv[u]) dv[u] = INFINITY;
+        } else {
+            ci[u] = INT_MAX;
+        }
+    }
+    int rank[2] = {0, 0};
+    for (int j = 0; j < 64; ++j) {
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            const float dj = __shfl(dv[v], j);
+            const int ij = __shfl(ci[v], j);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) rank[u] += key_less(dj, ij, j + 64 * v, dv[u], ci[u], lane + 64 * u) ? 1 : 0;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        if (valid && rank[u] >= 1 && rank[u] <= k) {
+            if (ci[u] == INT_MAX) atomicExch(err, 1);
+            inds[q * k + (k - rank[u])] = (ci[u] == INT_MAX) ? 0 : ci[u];
+            dists[q * k + (k - rank[u])] = dv[u];
+        }
+    }
+}
+
 // ------------------------------------------------------------------------ generic kNN
 // One thread per query: exact FP32 scan over all rows keeping a max-heap of the k+1 best
 // (distance, label) pairs in `heap` scratch ([N, k+1] pairs). All lanes of a wave visit the same
@@ -1117,6 +1160,32 @@ static void launch_tiles(int KL, int DK, int64_t N, int64_t Npad, const void *Xh
         launch_tiles_dk<P, QB, 32>(DK, grid, Xh, cn, N, Npad, beta, cand, st);
 }
 
+// k + 1 in 17..64 (lists of 32 or 64 per lane half): one query set per wave, since two sets'
+// lists do not fit in VGPRs next to the query fragments (16-bit: register-staged tiles; FP32:
+// the LDS-DMA tiles, which are one set per wave anyway).
+template <int P, int KL>
+static void launch_tiles_q1(int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
+                            int32_t *cand, hipStream_t st) {
+    dim3 grid((unsigned)(Npad / (4 * 32)));
+    if constexpr (P == PREC_FP32) {
+        switch (DK) {
+            case 1: k_knn_tiles_dma<P, 1, KL, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            case 2: k_knn_tiles_dma<P, 1, KL, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            case 4: k_knn_tiles_dma<P, 1, KL, 4><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            case 8: k_knn_tiles_dma<P, 1, KL, 8><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            default: throw HipError("[ncvis_amd] unsupported padded dimension");
+        }
+    } else {
+        switch (DK) {
+            case 1: k_knn_tiles<P, 1, KL, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            case 2: k_knn_tiles<P, 1, KL, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            case 4: k_knn_tiles<P, 1, KL, 4><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            case 8: k_knn_tiles<P, 1, KL, 8><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
+            default: throw HipError("[ncvis_amd] unsupported padded dimension");
+        }
+    }
+}
+
 // Rows of 257..512 elements (16-bit GEMM): one query set per wave, so the query fragments
 // (Dpad / 16 per lane) still fit in VGPRs; 32-row stages.
 template <int P, int KL>
@@ -1162,11 +1231,12 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
     // 16-bit GEMMs tile rows up to 512 elements, FP32 up to 256; wider rows or k + 1 > 32 take
     // the generic exact scan
     const bool wide = (prec != PREC_FP32) && (D > 256) && (D <= 512);
-    const bool tiled = ((D <= 256) || wide) && (k + 1 <= 32);
+    const int KL = (k + 1 <= 16) ? 16 : (k + 1 <= 32) ? 32 : 64;
+    const bool tiled = (D <= 256 && k + 1 <= 64) || (wide && k + 1 <= 32);
     if (tiled) {
         const int DK = D <= 32 ? 1 : D <= 64 ? 2 : D <= 128 ? 4 : D <= 256 ? 8 : D <= 320 ? 10 : D <= 384 ? 12 : 16;
         const int Dpad = 32 * DK;
-        const int QB = wide ? 1 : (prec == PREC_FP32) ? qb_for<PREC_FP32>() : qb_for<PREC_BF16>();
+        const int QB = (wide || KL >= 32) ? 1 : (prec == PREC_FP32) ? qb_for<PREC_FP32>() : qb_for<PREC_BF16>();
         const int64_t qtile = 4 * 32 * QB;
         const int64_t Npad = ((N + qtile - 1) / qtile) * qtile;
         const int esz = (prec == PREC_FP32) ? 4 : 2;
@@ -1185,11 +1255,21 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
             default: throw HipError("[ncvis_amd] knn: unknown precision");
         }
         NCVIS_LAUNCH_CHECK();
-        const int KL = (k + 1 <= 16) ? 16 : 32;
         DevBuf<int32_t> cand((size_t)N * 2 * KL);
         const float beta = is_l2 ? -2.0f : -1.0f;
         NCVIS_HIP_CHECK(hipEventRecord(e0, st));
-        if (wide) {
+        if (KL == 64 && !wide) {
+            switch (prec) {
+                case PREC_FP32: launch_tiles_q1<PREC_FP32, 64>(DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
+                case PREC_BF16: launch_tiles_q1<PREC_BF16, 64>(DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
+                default: launch_tiles_q1<PREC_FP16, 64>(DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
+            }
+        } else if (KL == 32 && !wide && prec != PREC_FP32) {
+            if (prec == PREC_BF16)
+                launch_tiles_q1<PREC_BF16, 32>(DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
+            else
+                launch_tiles_q1<PREC_FP16, 32>(DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
+        } else if (wide) {
             if (prec == PREC_BF16)
                 launch_tiles_wide<PREC_BF16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
             else
@@ -1206,8 +1286,10 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
         const int vec4 = (D % 4 == 0) && ((uintptr_t)Xp % 16 == 0);
         if (KL == 16)
             k_knn_rerank<32><<<cdiv(N, 8), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand.p, k, d_inds, d_dists, err.p);
-        else
+        else if (KL == 32)
             k_knn_rerank<64><<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand.p, k, d_inds, d_dists, err.p);
+        else
+            k_knn_rerank128<<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand.p, k, d_inds, d_dists, err.p);
         NCVIS_LAUNCH_CHECK();
         NCVIS_HIP_CHECK(hipEventRecord(e2, st));
     } else {

@@ -121,7 +121,8 @@ def test_init_embedding_matches_oracle(torch_cuda, n_init):
 # ------------------------------------------------------------------------------ kNN
 @pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
 @pytest.mark.parametrize("shape", [(1500, 64, 15), (700, 32, 5), (333, 100, 31), (40, 16, 15),
-                                   (600, 300, 15), (500, 257, 10), (300, 512, 31), (450, 384, 15)])
+                                   (600, 300, 15), (500, 257, 10), (300, 512, 31), (450, 384, 15),
+                                   (700, 64, 50), (400, 128, 40), (200, 32, 63), (300, 300, 45)])
 def test_knn_exact_on_integer_data(torch_cuda, precision, shape):
     # integer data: every precision computes exact candidate scores, so the table equals the
     # exact (distance, label) order bit for bit, ties included
