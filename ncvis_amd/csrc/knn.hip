@@ -18,7 +18,7 @@
 //                   sorted top-KL lists (two half-lists per query, one per lane half)
 //   k_knn_rerank  - exact FP32 distances for the 2*KL candidates of each query, rank by
 //                   (distance, label), drop rank 0, write ranks k..1 (farthest first)
-//   k_knn_generic - any D / any k: one thread per query, exact FP32 scan with a max-heap
+//   k_knn_generic - the remaining shapes: exact FP32 scan (LDS-tiled) with a max-heap per query
 #include <cfloat>
 #include <climits>
 
@@ -986,60 +986,115 @@ __global__ __launch_bounds__(256) void k_knn_rerank128(const float *__restrict__
 }
 
 // ------------------------------------------------------------------------ generic kNN
-// One thread per query: exact FP32 scan over all rows keeping a max-heap of the k+1 best
-// (distance, label) pairs in `heap` scratch ([N, k+1] pairs). All lanes of a wave visit the same
-// candidate row at the same time, so candidate reads are broadcasts.
+// Max-heap of the k+1 best (distance, label) pairs of one query ([k+1] in global scratch);
+// returns the new entry count.
+__device__ __noinline__ int heap_offer(float *hd, int *hi, int cnt, int K1, float dv, int ic) {
+    if (cnt < K1) {
+        int pos = cnt++;
+        while (pos > 0) {
+            int par = (pos - 1) >> 1;
+            if (key_less(hd[par], hi[par], 0, dv, ic, 0)) {
+                hd[pos] = hd[par];
+                hi[pos] = hi[par];
+                pos = par;
+            } else {
+                break;
+            }
+        }
+        hd[pos] = dv;
+        hi[pos] = ic;
+    } else if (key_less(dv, ic, 0, hd[0], hi[0], 0)) {
+        int pos = 0;
+        while (true) {
+            int l = 2 * pos + 1, r = l + 1, big = pos;
+            float bd = dv;
+            int bi = ic;
+            if (l < K1 && key_less(bd, bi, 0, hd[l], hi[l], 0)) {
+                big = l;
+                bd = hd[l];
+                bi = hi[l];
+            }
+            if (r < K1 && key_less(bd, bi, 0, hd[r], hi[r], 0)) big = r;
+            if (big == pos) break;
+            hd[pos] = hd[big];
+            hi[pos] = hi[big];
+            pos = big;
+        }
+        hd[pos] = dv;
+        hi[pos] = ic;
+    }
+    return cnt;
+}
+
+// Exact FP32 scan for the shapes the MFMA tiles do not take (wide FP32 rows, rows over 512,
+// k + 1 > 64): 256 queries per block, candidates in tiles of 64 and dimensions in chunks of 32
+// staged in LDS (candidate values read as broadcasts), 64 running sums per thread. Each
+// (query, candidate) sum runs over the dimensions in order with the same uncontracted operations
+// as exact_dist, so every distance is exact_dist's. The k+1 best (distance, label) pairs of a
+// query go through a max-heap, then the smallest is dropped and the rest written farthest first.
 __global__ __launch_bounds__(256) void k_knn_generic(const float *__restrict__ Xp, int64_t N, int D, int is_l2,
                                                      int k, float *__restrict__ hd_, int *__restrict__ hi_,
                                                      int32_t *__restrict__ inds, float *__restrict__ dists) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= N) return;
+    constexpr int TC = 64, DC = 32;
+    __shared__ float lq[256][DC + 1];
+    __shared__ __align__(16) float lc[DC][TC];
+    const int tid = threadIdx.x;
+    const int64_t qb = (int64_t)blockIdx.x * 256;
+    const int64_t q = qb + tid;
+    const bool valid = q < N;
     const int K1 = k + 1;
-    float *hd = hd_ + q * K1;
-    int *hi = hi_ + q * K1;
+    float *hd = hd_ + (valid ? q : 0) * K1;
+    int *hi = hi_ + (valid ? q : 0) * K1;
     int cnt = 0;
-    const float *xq = Xp + q * D;
-    for (int64_t c = 0; c < N; ++c) {
-        float dv = exact_dist(xq, Xp + c * D, D, is_l2);
-        if (dv != dv) dv = INFINITY;
-        int ic = (int)c;
-        if (cnt < K1) {
-            // sift up
-            int pos = cnt++;
-            while (pos > 0) {
-                int par = (pos - 1) >> 1;
-                if (key_less(hd[par], hi[par], 0, dv, ic, 0)) {
-                    hd[pos] = hd[par];
-                    hi[pos] = hi[par];
-                    pos = par;
-                } else {
-                    break;
+    for (int64_t c0 = 0; c0 < N; c0 += TC) {
+        float acc[TC];
+#pragma unroll
+        for (int t = 0; t < TC; ++t) acc[t] = 0.f;
+        for (int d0 = 0; d0 < D; d0 += DC) {
+            const int dn = (D - d0 < DC) ? D - d0 : DC;
+            __syncthreads();
+            for (int i = tid; i < 256 * DC; i += 256) {
+                const int r = i / DC, c = i - r * DC;
+                const int64_t row = qb + r;
+                lq[r][c] = (row < N && c < dn) ? Xp[row * D + d0 + c] : 0.f;
+            }
+            for (int i = tid; i < TC * DC; i += 256) {
+                const int t = i / DC, c = i - t * DC;
+                const int64_t row = c0 + t;
+                lc[c][t] = (row < N && c < dn) ? Xp[row * D + d0 + c] : 0.f;
+            }
+            __syncthreads();
+            for (int d = 0; d < dn; ++d) {
+                const float xq = lq[tid][d];
+#pragma unroll
+                for (int t4 = 0; t4 < TC; t4 += 4) {
+                    const float4 y = *(const float4 *)&lc[d][t4];
+                    const float yv[4] = {y.x, y.y, y.z, y.w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (is_l2) {
+                            const float e = __fsub_rn(xq, yv[u]);
+                            acc[t4 + u] = __fadd_rn(acc[t4 + u], __fmul_rn(e, e));
+                        } else {
+                            acc[t4 + u] = __fadd_rn(acc[t4 + u], __fmul_rn(xq, yv[u]));
+                        }
+                    }
                 }
             }
-            hd[pos] = dv;
-            hi[pos] = ic;
-        } else if (key_less(dv, ic, 0, hd[0], hi[0], 0)) {
-            // replace the max and sift down
-            int pos = 0;
-            while (true) {
-                int l = 2 * pos + 1, r = l + 1, big = pos;
-                float bd = dv;
-                int bi = ic;
-                if (l < K1 && key_less(bd, bi, 0, hd[l], hi[l], 0)) {
-                    big = l;
-                    bd = hd[l];
-                    bi = hi[l];
+        }
+        if (valid) {
+            const int tn = (N - c0 < TC) ? (int)(N - c0) : TC;
+#pragma unroll
+            for (int t = 0; t < TC; ++t) {
+                if (t < tn) {
+                    float dv = is_l2 ? acc[t] : __fsub_rn(1.0f, acc[t]);
+                    if (dv != dv) dv = INFINITY;
+                    cnt = heap_offer(hd, hi, cnt, K1, dv, (int)(c0 + t));
                 }
-                if (r < K1 && key_less(bd, bi, 0, hd[r], hi[r], 0)) big = r;
-                if (big == pos) break;
-                hd[pos] = hd[big];
-                hi[pos] = hi[big];
-                pos = big;
             }
-            hd[pos] = dv;
-            hi[pos] = ic;
         }
     }
+    if (!valid) return;
     // heap-sort ascending in place, then drop the smallest and write farthest first
     for (int end = cnt - 1; end > 0; --end) {
         float td = hd[0];
