@@ -343,3 +343,46 @@ def test_fit_transform_handle_device_api(torch_cuda):
     assert np.all(np.isfinite(dY.cpu().numpy()))
     with pytest.raises(RuntimeError, match="at least one element"):
         h.fit_transform_device(dX.data_ptr(), 0, 16, dY.data_ptr())
+
+
+# ------------------------------------------------------------------------------ tuning knobs
+_KNN_DEFAULTS = {"knn_skew": 1, "knn_stage_mult": 2, "knn_dma": -1, "knn_select": -1}
+
+
+@pytest.mark.parametrize("knobs", [{"knn_skew": 0}, {"knn_stage_mult": 1}, {"knn_skew": 0, "knn_stage_mult": 1},
+                                   {"knn_select": 1}, {"knn_dma": 1}, {"knn_dma": 0}])
+@pytest.mark.parametrize("precision", ["fp16", "fp32"])
+def test_knn_tuning_knobs_keep_the_table(torch_cuda, knobs, precision):
+    # every kernel variant a knob selects gives the exact table on integer data
+    from ncvis_amd import abi
+    _say(f"knn knobs {knobs} {precision}")
+    X, _ = synthetic.digits_like(N=1100, D=64, seed=21)
+    oi, od = oracle.knn_bruteforce(oracle.preprocess(X), 15)
+    try:
+        for key, v in knobs.items():
+            abi.tune(key, v)
+        gi, gd = gpu_knn(torch_cuda, X, 15, "euclidean", precision)
+    finally:
+        for key, v in _KNN_DEFAULTS.items():
+            abi.tune(key, v)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+
+
+@pytest.mark.parametrize("knobs", [{"sgd_prefetch": 4}, {"sgd_fast_math": 1}, {"max_chains": 64}])
+def test_sgd_tuning_knobs_run(torch_cuda, knobs):
+    # the many-chain epoch variants the SGD knobs select all descend the same objective
+    from ncvis_amd import abi
+    _say(f"sgd knobs {knobs}")
+    X, inds, dists, row_ptr, col = _digits_graph(N=1500, D=32, k=15, seed=5)
+    N = X.shape[0]
+    Y0 = oracle.init_embedding(N, 2, row_ptr, col, np.float32(1.0 / 15), 5, 42, 1)
+    defaults = {"sgd_prefetch": 1, "sgd_fast_math": 0, "max_chains": 1 << 17}
+    try:
+        for key, v in knobs.items():
+            abi.tune(key, v)
+        Y, q, _ = gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, [3] * 20, A_DEF, B_DEF, 1.0, 1.0, 42, 0, 1)
+    finally:
+        for key, v in defaults.items():
+            abi.tune(key, v)
+    assert np.all(np.isfinite(Y)) and np.isfinite(q)
+    assert Y.std() > 0.5 * Y0.std()
