@@ -64,8 +64,8 @@ int ncvis_set_option(ncvis_handle *h, const char *key, double value);
  * knobs never change the kNN table. Keys: "sgd_prefetch" (1,2,4,8), "sgd_plain_far" (0/1),
  * "sgd_fast_math" (0/1), "edges_per_chain" (auto chain count = E / this, at least
  * min(E / 16, "min_chains"), at most "max_chains"),
- * "knn_dma" (-1 auto, 0, 1), "knn_select" (-1 auto, 0 register lists, 1 buffered),
- * "knn_stage_mult" (1, 2), "knn_skew" (0/1: skewed two-set 16-bit tile kernel). */
+ * "knn_dma" (FP32 tiles: -1 auto = 1 LDS DMA, 0 register-staged), "knn_stage_mult" (16-bit
+ * tiles: 1 or 2 x the LDS stage rows). */
 int ncvis_tune(const char *key, double value);
 
 /* ncvis::NCVis::fit_transform: X [N, D] and Y [N, d] are host, row-major float32. */

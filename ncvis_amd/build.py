@@ -23,7 +23,10 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libncvis_amd.so")
 
-HIP_SOURCES = ["knn.hip", "symmetrize.hip", "embed.hip", "abi.hip"]
+# the kNN tile instantiations are split over several sources so they compile in parallel
+KNN_TILE_SOURCES = ["knn_q1_f32dma.hip", "knn_q1_f32.hip", "knn_q1_bf16.hip", "knn_q1_fp16.hip", "knn_t16.hip",
+                    "knn_wide.hip"]
+HIP_SOURCES = KNN_TILE_SOURCES + ["knn.hip", "symmetrize.hip", "embed.hip", "abi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -33,7 +36,7 @@ HIP_FLAGS = [
 ]
 # per-source extras: the kNN tiles keep MFMA accumulators in VGPRs (no AGPR copies before the
 # VALU scoring of every block)
-HIP_EXTRA = {"knn.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+HIP_EXTRA = {src: ["-mllvm", "-amdgpu-mfma-vgpr-form"] for src in KNN_TILE_SOURCES + ["knn.hip"]}
 
 
 def _run(cmd):
@@ -54,7 +57,7 @@ def build_hip(verbose: bool = False) -> str:
         _run([HIPCC, *HIP_FLAGS, *HIP_EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj])
         return obj
 
-    with ThreadPoolExecutor(max_workers=len(HIP_SOURCES)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(HIP_SOURCES), max(2, os.cpu_count() or 2))) as ex:
         objs = list(ex.map(compile_one, HIP_SOURCES))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])

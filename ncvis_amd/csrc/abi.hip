@@ -244,13 +244,9 @@ int ncvis_tune(const char *key, double value) {
             t.edges_per_chain = (int64_t)value;
         } else if (k == "knn_dma") {
             t.knn_dma = value < 0 ? -1 : (value != 0);
-        } else if (k == "knn_skew") {
-            t.knn_skew = value != 0 ? 1 : 0;
         } else if (k == "knn_stage_mult") {
             if (value != 1 && value != 2) throw std::invalid_argument("[ncvis_amd] knn_stage_mult: 1 or 2");
             t.knn_stage_mult = (int)value;
-        } else if (k == "knn_select") {
-            t.knn_select = value < 0 ? -1 : (value != 0);
         } else if (k == "min_chains") {
             if (value < 1) throw std::invalid_argument("[ncvis_amd] min_chains must be >= 1");
             t.min_chains = (int64_t)value;

@@ -14,10 +14,8 @@ struct Tuning {
     int64_t edges_per_chain = 64;    // auto chain count = E / edges_per_chain,
     int64_t min_chains = 1;          // ... at least min(E / 16, min_chains) (1: off) ...
     int64_t max_chains = 1 << 17;    // ... and at most this
-    int knn_dma = -1;                // kNN tiles fetch stages with LDS DMA (1), via registers (0), auto (-1)
-    int knn_select = -1;             // kNN top-k: register lists (0), buffered in HBM (1), auto (-1)
-    int knn_stage_mult = 2;          // kNN LDS stage rows x1 or x2 (register-staged kernels)
-    int knn_skew = 1;                // 16-bit kNN: skewed two-set tile kernel (1) or plain (0)
+    int knn_dma = -1;                // FP32 kNN tiles: stages by LDS DMA (1 or auto -1) or registers (0)
+    int knn_stage_mult = 2;          // 16-bit kNN (k + 1 <= 16): LDS stage rows x1 or x2
 };
 Tuning &tuning();
 
@@ -33,6 +31,22 @@ struct KnnStats {
 };
 void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32_t *d_inds, float *d_dists,
              hipStream_t st, KnnStats *stats = nullptr);
+
+// Candidate-tile launches on the packed matrix Xh [Npad, 32*DK] and offsets cn [Npad], writing the
+// two lane half-lists of KL ids per query to cand [N, 2*KL] (one translation unit each).
+void knn_tiles16(int prec, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
+                 int32_t *cand, hipStream_t st);  // KL = 16 (knn_t16.hip)
+// KL = 32 or 64, one translation unit per GEMM precision / FP32 staging (knn_q1_*.hip)
+void knn_tiles_q1_f32dma(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
+                         int32_t *cand, hipStream_t st);
+void knn_tiles_q1_f32(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
+                      int32_t *cand, hipStream_t st);
+void knn_tiles_q1_bf16(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
+                       int32_t *cand, hipStream_t st);
+void knn_tiles_q1_fp16(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
+                       int32_t *cand, hipStream_t st);
+void knn_tiles_wide(int prec, int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn,
+                    float beta, int32_t *cand, hipStream_t st);  // 16-bit, DK 10/12/16 (knn_wide.hip)
 
 // ---- symmetrize (symmetrize.hip) ---------------------------------------------------------
 // KNNTable::symmetrize at one thread, bit-exact, then flattened to CSR (= build_edges order).

@@ -12,10 +12,11 @@
 //   k_pack        - candidate matrix in the GEMM precision (FP32, BF16 or FP16), zero padded to
 //                   Dpad = 32*DK columns, plus per-row score offsets cn (||x||^2 for L2, 0 for IP,
 //                   +inf for padding rows)
-//   k_knn_tiles   - MFMA tiles: a workgroup owns 4 waves x 32*QB queries whose B fragments live
-//                   in registers; candidate rows stream through double-buffered LDS stages; every
-//                   32x32 MFMA block is scored (cn + beta * <q, c>) and filtered into per-lane
-//                   sorted top-KL lists (two half-lists per query, one per lane half)
+//   k_knn_tiles   - MFMA tiles (knn_tiles.h, instantiated in knn_t16 / knn_q1_* / knn_wide): a
+//                   workgroup owns 4 waves x 32*QB queries whose B fragments live in registers;
+//                   candidate rows stream through double-buffered LDS stages; every 32x32 MFMA
+//                   block is scored (cn + beta * <q, c>) and filtered into per-lane sorted top-KL
+//                   lists (two half-lists per query, one per lane half)
 //   k_knn_rerank  - exact FP32 distances for the 2*KL candidates of each query, rank by
 //                   (distance, label), drop rank 0, write ranks k..1 (farthest first)
 //   k_knn_generic - the remaining shapes: exact FP32 scan (LDS-tiled) with a max-heap per query
@@ -26,10 +27,6 @@
 #include "kernels.h"
 
 namespace ncvis_hip {
-
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
-typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 // ------------------------------------------------------------------------ preprocess
 // src/ncvis.cpp:32-70; sums are sequential and uncontracted like the reference's loops.
@@ -103,759 +100,6 @@ __global__ void k_pack(const float *__restrict__ Xp, int64_t N, int64_t Npad, in
         }
     }
     cn[i] = (i < N) ? (is_l2 ? s : 0.f) : INFINITY;
-}
-
-// ------------------------------------------------------------------------ top-k lists
-// Sorted ascending list: x goes to the first slot p with x < ld[p], the tail shifts down one slot
-// and the last entry drops out. Among equal scores the earlier entry stays first and the later
-// one drops first, so with candidates visited in ascending label order a list holds the KL
-// smallest (score, label) pairs. Every new slot is a function of the old list only: the score
-// of slot p is med3(x, ld[p-1], ld[p]) (the list is sorted), its label comes from slot p-1, x or
-// itself, so the update runs in place without swaps. Requires x < ld[KL-1] (x not NaN).
-template <int KL>
-__device__ __forceinline__ void list_insert(float (&ld)[KL], int (&li)[KL], float x, int xi) {
-    bool c[KL];
-#pragma unroll
-    for (int p = 0; p < KL; ++p) c[p] = x < ld[p];
-#pragma unroll
-    for (int p = KL - 1; p >= 1; --p) {
-        ld[p] = __builtin_amdgcn_fmed3f(x, ld[p - 1], ld[p]);
-        li[p] = c[p - 1] ? li[p - 1] : (c[p] ? xi : li[p]);
-    }
-    ld[0] = c[0] ? x : ld[0];
-    li[0] = c[0] ? xi : li[0];
-}
-
-template <int P>
-struct Frag;
-template <>
-struct Frag<PREC_FP32> {
-    using T = float;  // one k element per lane per 32x32x2 step
-    static constexpr int ESZ = 4;
-};
-template <>
-struct Frag<PREC_BF16> {
-    using T = uint4;  // eight bf16 per lane per 32x32x16 step
-    static constexpr int ESZ = 2;
-};
-template <>
-struct Frag<PREC_FP16> {
-    using T = uint4;
-    static constexpr int ESZ = 2;
-};
-
-template <int P>
-__device__ __forceinline__ f32x16_t mfma_step(typename Frag<P>::T a, typename Frag<P>::T b, f32x16_t c) {
-    if constexpr (P == PREC_FP32) {
-        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
-    } else if constexpr (P == PREC_BF16) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a),
-                                                       __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
-    } else {
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8_t, a),
-                                                      __builtin_bit_cast(f16x8_t, b), c, 0, 0, 0);
-    }
-}
-
-// MFMA 32x32 output layout: lane l holds column l%32 and rows 8*(i/4) + 4*(l/32) + (i%4),
-// i = 0..15. Candidates are the A rows, queries the B columns, so a lane holds one query.
-template <int P, int QB, int KL, int DK, int SM = 1>
-__global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_, const float *__restrict__ cn,
-                                                   int64_t N, int64_t Npad, float beta,
-                                                   int32_t *__restrict__ cand) {
-    using FT = typename Frag<P>::T;
-    constexpr int ESZ = Frag<P>::ESZ;
-    constexpr int DP = 32 * DK;                       // padded row length (elements)
-    constexpr int KS = (P == PREC_FP32) ? DP / 2 : DP / 16;  // MFMA k-steps per row
-    constexpr int ROWB = DP * ESZ;                    // row bytes in HBM
-    constexpr int CB = ((ROWB >= 512) ? 32 : 64) * SM;  // candidate rows per LDS stage
-    constexpr int LROWB = ROWB + 16;                  // padded LDS row (bank spread)
-    constexpr int NV = CB * ROWB / 16 / 256;          // uint4 per thread per stage
-    constexpr int CHUNKS = ROWB / 16;                 // uint4 per row
-    static_assert(NV >= 1 && NV * 256 * 16 == CB * ROWB, "stage must split evenly over 256 threads");
-
-    __shared__ __align__(16) unsigned char lds[2][CB * LROWB];
-    __shared__ __align__(16) float ldc[2][CB];  // the stage's score offsets travel with its rows
-
-    const unsigned char *Xh = (const unsigned char *)Xh_;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int h = lane >> 5;
-    const int col = lane & 31;
-    const int64_t q0 = ((int64_t)blockIdx.x * 4 + wave) * (32 * QB);
-
-    // query (B) fragments, resident for the whole candidate sweep
-    FT qf[QB][KS];
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-        const unsigned char *row = Xh + (q0 + qb * 32 + col) * (int64_t)ROWB;
-        if constexpr (P == PREC_FP32) {
-#pragma unroll
-            for (int s = 0; s < KS; s += 4) {
-                float4 v = *(const float4 *)(row + (h * (DP / 2) + s) * 4);
-                qf[qb][s] = v.x;
-                qf[qb][s + 1] = v.y;
-                qf[qb][s + 2] = v.z;
-                qf[qb][s + 3] = v.w;
-            }
-        } else {
-#pragma unroll
-            for (int s = 0; s < KS; ++s) qf[qb][s] = *(const uint4 *)(row + s * 32 + h * 16);
-        }
-    }
-
-    float ld[QB][KL];
-    int li[QB][KL];
-    float thr[QB];
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-#pragma unroll
-        for (int p = 0; p < KL; ++p) {
-            ld[qb][p] = INFINITY;
-            li[qb][p] = -1;
-        }
-        thr[qb] = INFINITY;
-    }
-
-    const int64_t nstages = Npad / CB;
-    const bool cn_lane = threadIdx.x < CB / 4;  // these threads carry the stage's cn (float4 each)
-    uint4 stage[NV];
-    float4 stage_cn = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(Xh + (int64_t)(v * 256 + threadIdx.x) * 16);
-    if (cn_lane) stage_cn = *(const float4 *)(cn + 4 * threadIdx.x);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        int u = v * 256 + threadIdx.x;
-        int r = u / CHUNKS, c = u - r * CHUNKS;
-        *(uint4 *)(&lds[0][r * LROWB + c * 16]) = stage[v];
-    }
-    if (cn_lane) *(float4 *)(&ldc[0][4 * threadIdx.x]) = stage_cn;
-    __syncthreads();
-
-    for (int64_t st = 0; st < nstages; ++st) {
-        const int buf = (int)(st & 1);
-        {
-            // next stage (the last iteration reloads its own stage; the copy is never read)
-            const int64_t nx = (st + 1 < nstages) ? st + 1 : st;
-            const unsigned char *nb = Xh + nx * CB * (int64_t)ROWB;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(nb + (int64_t)(v * 256 + threadIdx.x) * 16);
-            if (cn_lane) stage_cn = *(const float4 *)(cn + nx * CB + 4 * threadIdx.x);
-        }
-#pragma unroll
-        for (int rb = 0; rb < CB / 32; ++rb) {
-            const int64_t c0 = st * CB + rb * 32;
-            const unsigned char *arow = &lds[buf][(rb * 32 + col) * LROWB];
-            f32x16_t acc[QB];
-#pragma unroll
-            for (int qb = 0; qb < QB; ++qb) acc[qb] = f32x16_t{};
-            if constexpr (P == PREC_FP32) {
-#pragma unroll
-                for (int s = 0; s < KS; s += 4) {
-                    float4 a4 = *(const float4 *)(arow + (h * (DP / 2) + s) * 4);
-#pragma unroll
-                    for (int qb = 0; qb < QB; ++qb) {
-                        acc[qb] = mfma_step<P>(a4.x, qf[qb][s], acc[qb]);
-                        acc[qb] = mfma_step<P>(a4.y, qf[qb][s + 1], acc[qb]);
-                        acc[qb] = mfma_step<P>(a4.z, qf[qb][s + 2], acc[qb]);
-                        acc[qb] = mfma_step<P>(a4.w, qf[qb][s + 3], acc[qb]);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int s = 0; s < KS; ++s) {
-                    uint4 a = *(const uint4 *)(arow + s * 32 + h * 16);
-#pragma unroll
-                    for (int qb = 0; qb < QB; ++qb) acc[qb] = mfma_step<P>(a, qf[qb][s], acc[qb]);
-                }
-            }
-            // score offsets of this lane's 16 candidate rows: c0 + 8g + 4h + j
-            float cnv[16];
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                float4 v = *(const float4 *)(&ldc[buf][rb * 32 + 8 * g + 4 * h]);
-                cnv[4 * g] = v.x;
-                cnv[4 * g + 1] = v.y;
-                cnv[4 * g + 2] = v.z;
-                cnv[4 * g + 3] = v.w;
-            }
-#pragma unroll
-            for (int qb = 0; qb < QB; ++qb) {
-                float m = INFINITY;
-#pragma unroll
-                for (int i = 0; i < 16; ++i) m = fminf(m, fmaf(beta, acc[qb][i], cnv[i]));
-                if (__any(m < thr[qb])) {
-#pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        float sc = fmaf(beta, acc[qb][i], cnv[i]);
-                        bool pass = sc < thr[qb];
-                        if (__any(pass)) {
-                            if (pass) {
-                                int ci = (int)(c0 + 8 * (i >> 2) + 4 * h + (i & 3));
-                                list_insert<KL>(ld[qb], li[qb], sc, ci);
-                                thr[qb] = ld[qb][KL - 1];
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                int u = v * 256 + threadIdx.x;
-                int r = u / CHUNKS, c = u - r * CHUNKS;
-                *(uint4 *)(&lds[buf ^ 1][r * LROWB + c * 16]) = stage[v];
-            }
-            if (cn_lane) *(float4 *)(&ldc[buf ^ 1][4 * threadIdx.x]) = stage_cn;
-        }
-        __syncthreads();
-    }
-
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-        int64_t q = q0 + qb * 32 + col;
-        if (q < N) {
-            int32_t *o = cand + q * (2 * KL) + h * KL;
-#pragma unroll
-            for (int p = 0; p < KL; ++p) o[p] = li[qb][p];
-        }
-    }
-}
-
-// ------------------------------------------------------------------------ skewed two-set tiles
-// Scores of one 32x32 block (held in the accumulator, overwritten in place by cn + beta * dot)
-// filtered into a lane's sorted list; candidates are visited in ascending label order.
-template <int KL>
-__device__ __forceinline__ void block_insert(float (&ld)[KL], int (&li)[KL], float &thr, const f32x16_t &sc,
-                                             int64_t c0, int h) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const bool pass = sc[i] < thr;
-        if (__any(pass)) {
-            if (pass) {
-                list_insert<KL>(ld, li, sc[i], (int)(c0 + 8 * (i >> 2) + 4 * h + (i & 3)));
-                thr = ld[KL - 1];
-            }
-        }
-    }
-}
-
-// cn + beta * dot for the lane's 16 rows of a block (offsets read from the stage's LDS copy);
-// returns the lane's smallest score.
-__device__ __forceinline__ float block_scores(f32x16_t &acc, const float *pc, int h, float beta) {
-    float m = INFINITY;
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        const float4 v = *(const float4 *)(pc + 8 * g + 4 * h);
-        acc[4 * g] = fmaf(beta, acc[4 * g], v.x);
-        acc[4 * g + 1] = fmaf(beta, acc[4 * g + 1], v.y);
-        acc[4 * g + 2] = fmaf(beta, acc[4 * g + 2], v.z);
-        acc[4 * g + 3] = fmaf(beta, acc[4 * g + 3], v.w);
-        m = fminf(m, fminf(fminf(acc[4 * g], acc[4 * g + 1]), fminf(acc[4 * g + 2], acc[4 * g + 3])));
-    }
-    return m;
-}
-
-// Same tiles, lists and visiting order as k_knn_tiles at QB = 2 (16-bit GEMM), with the two query
-// sets of a wave skewed by one block: the MFMA chain of set 0 for block b is issued together
-// with the scoring of set 1 for block b-1, and the chain of set 1 for block b together with the
-// scoring of set 0 for block b, so one wave keeps its MFMA pipe busy while its VALU filters the
-// other set. Score offsets are triple-buffered in LDS because set 1 reads the previous stage's
-// offsets after the stage barrier. The candidate lists equal k_knn_tiles' bit for bit.
-template <int P, int KL, int DK, int SM>
-__global__ __launch_bounds__(256) void k_knn_tiles_sk(const void *__restrict__ Xh_, const float *__restrict__ cn,
-                                                      int64_t N, int64_t Npad, float beta,
-                                                      int32_t *__restrict__ cand) {
-    static_assert(P != PREC_FP32, "16-bit candidate GEMM only");
-    constexpr int QB = 2;
-    constexpr int ESZ = 2;
-    constexpr int DP = 32 * DK;
-    constexpr int KS = DP / 16;
-    constexpr int ROWB = DP * ESZ;
-    constexpr int CB = ((ROWB >= 512) ? 32 : 64) * SM;
-    constexpr int LROWB = ROWB + 16;
-    constexpr int NV = CB * ROWB / 16 / 256;
-    constexpr int CHUNKS = ROWB / 16;
-    constexpr int NB = CB / 32;
-    static_assert(NV >= 1 && NV * 256 * 16 == CB * ROWB, "stage must split evenly over 256 threads");
-
-    __shared__ __align__(16) unsigned char lds[2][CB * LROWB];
-    __shared__ __align__(16) float ldc[3][CB];
-    __shared__ __align__(16) float ldinf[32];
-
-    const unsigned char *Xh = (const unsigned char *)Xh_;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int h = lane >> 5;
-    const int col = lane & 31;
-    const int64_t q0 = ((int64_t)blockIdx.x * 4 + wave) * (32 * QB);
-
-    uint4 qf[QB][KS];
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-        const unsigned char *row = Xh + (q0 + qb * 32 + col) * (int64_t)ROWB;
-#pragma unroll
-        for (int s = 0; s < KS; ++s) qf[qb][s] = *(const uint4 *)(row + s * 32 + h * 16);
-    }
-    float ld[QB][KL];
-    int li[QB][KL];
-    float thr[QB];
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-#pragma unroll
-        for (int p = 0; p < KL; ++p) {
-            ld[qb][p] = INFINITY;
-            li[qb][p] = -1;
-        }
-        thr[qb] = INFINITY;
-    }
-
-    const int64_t nstages = Npad / CB;
-    const bool cn_lane = threadIdx.x < CB / 4;
-    uint4 stage[NV];
-    float4 stage_cn = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(Xh + (int64_t)(v * 256 + threadIdx.x) * 16);
-    if (cn_lane) stage_cn = *(const float4 *)(cn + 4 * threadIdx.x);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        int u = v * 256 + threadIdx.x;
-        int r = u / CHUNKS, c = u - r * CHUNKS;
-        *(uint4 *)(&lds[0][r * LROWB + c * 16]) = stage[v];
-    }
-    if (cn_lane) *(float4 *)(&ldc[0][4 * threadIdx.x]) = stage_cn;
-    if (threadIdx.x < 32) ldinf[threadIdx.x] = INFINITY;
-    __syncthreads();
-
-    // set 1's accumulator of the previous block; before the first block it scores +inf
-    f32x16_t acc1 = f32x16_t{};
-    const float *pc1 = ldinf;
-    int64_t pc0 = 0;
-    int cb = 0;  // st % 3
-    for (int64_t st = 0; st < nstages; ++st) {
-        const int buf = (int)(st & 1);
-        {
-            const int64_t nx = (st + 1 < nstages) ? st + 1 : st;
-            const unsigned char *nb = Xh + nx * CB * (int64_t)ROWB;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(nb + (int64_t)(v * 256 + threadIdx.x) * 16);
-            if (cn_lane) stage_cn = *(const float4 *)(cn + nx * CB + 4 * threadIdx.x);
-        }
-#pragma unroll
-        for (int rb = 0; rb < NB; ++rb) {
-            const int64_t c0 = st * CB + rb * 32;
-            const unsigned char *arow = &lds[buf][(rb * 32 + col) * LROWB];
-            uint4 a[KS];
-#pragma unroll
-            for (int s = 0; s < KS; ++s) a[s] = *(const uint4 *)(arow + s * 32 + h * 16);
-            f32x16_t acc0 = f32x16_t{};
-#pragma unroll
-            for (int s = 0; s < KS; ++s) acc0 = mfma_step<P>(a[s], qf[0][s], acc0);
-            const float m1 = block_scores(acc1, pc1, h, beta);
-            if (__any(m1 < thr[1])) block_insert<KL>(ld[1], li[1], thr[1], acc1, pc0, h);
-            acc1 = f32x16_t{};
-#pragma unroll
-            for (int s = 0; s < KS; ++s) acc1 = mfma_step<P>(a[s], qf[1][s], acc1);
-            const float *pc = &ldc[cb][rb * 32];
-            const float m0 = block_scores(acc0, pc, h, beta);
-            if (__any(m0 < thr[0])) block_insert<KL>(ld[0], li[0], thr[0], acc0, c0, h);
-            pc1 = pc;
-            pc0 = c0;
-        }
-        const int cbn = (cb == 2) ? 0 : cb + 1;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            int u = v * 256 + threadIdx.x;
-            int r = u / CHUNKS, c = u - r * CHUNKS;
-            *(uint4 *)(&lds[buf ^ 1][r * LROWB + c * 16]) = stage[v];
-        }
-        if (cn_lane) *(float4 *)(&ldc[cbn][4 * threadIdx.x]) = stage_cn;
-        cb = cbn;
-        __syncthreads();
-    }
-    {
-        const float m1 = block_scores(acc1, pc1, h, beta);
-        if (__any(m1 < thr[1])) block_insert<KL>(ld[1], li[1], thr[1], acc1, pc0, h);
-    }
-
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-        int64_t q = q0 + qb * 32 + col;
-        if (q < N) {
-            int32_t *o = cand + q * (2 * KL) + h * KL;
-#pragma unroll
-            for (int p = 0; p < KL; ++p) o[p] = li[qb][p];
-        }
-    }
-}
-
-// ------------------------------------------------------------------------ buffered selection
-// Variant of k_knn_tiles whose hot loop holds no list: each (query, lane half) keeps only a
-// threshold and appends candidates that beat it, {score bits, label}, to its own buffer of CAP
-// entries in HBM (a few hundred appends over a whole sweep). When a buffer could overflow, the
-// wave re-selects the best KL of every buffer in lockstep (one sorted-insertion pass per buffered
-// slot, registers live only inside that pass) and tightens the thresholds. Entries are appended
-// in ascending label order and re-selected in buffer order with strict '<', so the final lists
-// equal one-at-a-time insertion.
-template <int KL>
-__device__ __forceinline__ void buf_reselect(uint2 *__restrict__ b, int &cnt, float &thr) {
-    float ld[KL];
-    int li[KL];
-#pragma unroll
-    for (int p = 0; p < KL; ++p) {
-        ld[p] = INFINITY;
-        li[p] = -1;
-    }
-    for (int j = 0; __any(j < cnt); ++j) {
-        const uint2 e = (j < cnt) ? b[j] : make_uint2(0x7f800000u, 0u);
-        list_insert<KL>(ld, li, __uint_as_float(e.x), (int)e.y);
-    }
-    int nvalid = 0;
-#pragma unroll
-    for (int p = 0; p < KL; ++p) {
-        b[p] = make_uint2(__float_as_uint(ld[p]), (unsigned)li[p]);
-        nvalid += (li[p] >= 0) ? 1 : 0;
-    }
-    cnt = nvalid;
-    thr = (nvalid == KL) ? ld[KL - 1] : INFINITY;
-}
-
-template <int P, int QB, int KL, int DK>
-__global__ __launch_bounds__(256) void k_knn_tiles_buf(const void *__restrict__ Xh_, const float *__restrict__ cn,
-                                                       int64_t N, int64_t Npad, float beta,
-                                                       uint2 *__restrict__ sbuf, int32_t *__restrict__ cand) {
-    using FT = typename Frag<P>::T;
-    constexpr int ESZ = Frag<P>::ESZ;
-    constexpr int DP = 32 * DK;
-    constexpr int KS = (P == PREC_FP32) ? DP / 2 : DP / 16;
-    constexpr int ROWB = DP * ESZ;
-    constexpr int CB = (ROWB >= 512) ? 32 : 64;
-    constexpr int LROWB = ROWB + 16;
-    constexpr int NV = CB * ROWB / 16 / 256;
-    constexpr int CHUNKS = ROWB / 16;
-    constexpr int CAP = 4 * KL;
-    static_assert(NV >= 1 && NV * 256 * 16 == CB * ROWB, "stage must split evenly over 256 threads");
-
-    __shared__ __align__(16) unsigned char lds[2][CB * LROWB];
-
-    const unsigned char *Xh = (const unsigned char *)Xh_;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int h = lane >> 5;
-    const int col = lane & 31;
-    const int64_t q0 = ((int64_t)blockIdx.x * 4 + wave) * (32 * QB);
-
-    FT qf[QB][KS];
-    uint2 *mb[QB];
-    float thr[QB];
-    int cnt[QB];
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-        const int64_t q = q0 + qb * 32 + col;
-        const unsigned char *row = Xh + q * (int64_t)ROWB;
-        if constexpr (P == PREC_FP32) {
-#pragma unroll
-            for (int s = 0; s < KS; s += 4) {
-                float4 v = *(const float4 *)(row + (h * (DP / 2) + s) * 4);
-                qf[qb][s] = v.x;
-                qf[qb][s + 1] = v.y;
-                qf[qb][s + 2] = v.z;
-                qf[qb][s + 3] = v.w;
-            }
-        } else {
-#pragma unroll
-            for (int s = 0; s < KS; ++s) qf[qb][s] = *(const uint4 *)(row + s * 32 + h * 16);
-        }
-        mb[qb] = sbuf + (q * 2 + h) * (int64_t)CAP;
-        thr[qb] = INFINITY;
-        cnt[qb] = 0;
-    }
-
-    const int64_t nstages = Npad / CB;
-    uint4 stage[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(Xh + (int64_t)(v * 256 + threadIdx.x) * 16);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        int u = v * 256 + threadIdx.x;
-        int r = u / CHUNKS, c = u - r * CHUNKS;
-        *(uint4 *)(&lds[0][r * LROWB + c * 16]) = stage[v];
-    }
-    __syncthreads();
-
-    for (int64_t st = 0; st < nstages; ++st) {
-        const int buf = (int)(st & 1);
-        {
-            const int64_t nx = (st + 1 < nstages) ? st + 1 : st;
-            const unsigned char *nb = Xh + nx * CB * (int64_t)ROWB;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(nb + (int64_t)(v * 256 + threadIdx.x) * 16);
-        }
-#pragma unroll
-        for (int rb = 0; rb < CB / 32; ++rb) {
-            const int64_t c0 = st * CB + rb * 32;
-            const unsigned char *arow = &lds[buf][(rb * 32 + col) * LROWB];
-            f32x16_t acc[QB];
-#pragma unroll
-            for (int qb = 0; qb < QB; ++qb) acc[qb] = f32x16_t{};
-            if constexpr (P == PREC_FP32) {
-#pragma unroll
-                for (int s = 0; s < KS; s += 4) {
-                    float4 a4 = *(const float4 *)(arow + (h * (DP / 2) + s) * 4);
-#pragma unroll
-                    for (int qb = 0; qb < QB; ++qb) {
-                        acc[qb] = mfma_step<P>(a4.x, qf[qb][s], acc[qb]);
-                        acc[qb] = mfma_step<P>(a4.y, qf[qb][s + 1], acc[qb]);
-                        acc[qb] = mfma_step<P>(a4.z, qf[qb][s + 2], acc[qb]);
-                        acc[qb] = mfma_step<P>(a4.w, qf[qb][s + 3], acc[qb]);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int s = 0; s < KS; ++s) {
-                    uint4 a = *(const uint4 *)(arow + s * 32 + h * 16);
-#pragma unroll
-                    for (int qb = 0; qb < QB; ++qb) acc[qb] = mfma_step<P>(a, qf[qb][s], acc[qb]);
-                }
-            }
-            float cnv[16];
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                float4 v = *(const float4 *)(cn + c0 + 8 * g + 4 * h);
-                cnv[4 * g] = v.x;
-                cnv[4 * g + 1] = v.y;
-                cnv[4 * g + 2] = v.z;
-                cnv[4 * g + 3] = v.w;
-            }
-#pragma unroll
-            for (int qb = 0; qb < QB; ++qb) {
-                float m = INFINITY;
-#pragma unroll
-                for (int i = 0; i < 16; ++i) m = fminf(m, fmaf(beta, acc[qb][i], cnv[i]));
-                if (__any(m < thr[qb])) {
-                    if (__any(cnt[qb] > CAP - 16)) buf_reselect<KL>(mb[qb], cnt[qb], thr[qb]);
-#pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        const float sc = fmaf(beta, acc[qb][i], cnv[i]);
-                        if (sc < thr[qb]) {
-                            mb[qb][cnt[qb]] = make_uint2(__float_as_uint(sc), (unsigned)(c0 + 8 * (i >> 2) + 4 * h + (i & 3)));
-                            ++cnt[qb];
-                        }
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            int u = v * 256 + threadIdx.x;
-            int r = u / CHUNKS, c = u - r * CHUNKS;
-            *(uint4 *)(&lds[buf ^ 1][r * LROWB + c * 16]) = stage[v];
-        }
-        __syncthreads();
-    }
-
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-        buf_reselect<KL>(mb[qb], cnt[qb], thr[qb]);
-        const int64_t q = q0 + qb * 32 + col;
-        if (q < N) {
-            int32_t *o = cand + q * (2 * KL) + h * KL;
-#pragma unroll
-            for (int p = 0; p < KL; ++p) o[p] = (int32_t)mb[qb][p].y;
-        }
-    }
-}
-
-// ------------------------------------------------------------------------ LDS-DMA tile kernel
-// Same tile and selection as k_knn_tiles, but candidate stages are fetched with
-// global_load_lds_dwordx4 straight into LDS (no register staging), three stages in flight. LDS
-// rows are unpadded; the 16-byte chunk c of row r sits in slot c ^ (r % CHUNKS), so the
-// fragment reads of 32 consecutive rows at one chunk index hit distinct banks. Each wave keeps
-// its own copy of a stage's score offsets, fetched by its first CB/4 lanes.
-#define NCVIS_AS1(p) ((const void __attribute__((address_space(1))) *)(p))
-#define NCVIS_AS3(p) ((void __attribute__((address_space(3))) *)(p))
-
-template <int NV, int CB, int ROWB, int CHUNKS>
-__device__ __forceinline__ void dma_stage(const unsigned char *__restrict__ Xh, const float *__restrict__ cn,
-                                          int64_t stage, unsigned char *lbuf, float *lcn_wave, int wave, int lane) {
-    const unsigned char *base = Xh + stage * (int64_t)(CB * ROWB);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        const int p = (v * 4 + wave) * 64 + lane;  // 16-byte LDS slot this lane fills
-        const int r = p / CHUNKS, s = p % CHUNKS;
-        const int c = s ^ (r & (CHUNKS - 1));
-        __builtin_amdgcn_global_load_lds(NCVIS_AS1(base + r * ROWB + c * 16), NCVIS_AS3(lbuf + (v * 4 + wave) * 1024),
-                                         16, 0, 0);
-    }
-    if (lane < CB / 4)
-        __builtin_amdgcn_global_load_lds(NCVIS_AS1(cn + stage * CB + 4 * lane), NCVIS_AS3(lcn_wave), 16, 0, 0);
-}
-
-// s_waitcnt with only the vector-memory counter constrained (gfx9 encoding)
-#define NCVIS_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
-
-// Workgroup barrier without a memory fence: the stage pipelines order their LDS transfers with
-// explicit vector-memory waits, and a fenced barrier would also wait for the transfers of the
-// stages still in flight. The asm "memory" clobber keeps the compiler from moving LDS accesses
-// across it.
-#define NCVIS_BARRIER_NOFENCE() asm volatile("s_barrier" ::: "memory")
-
-template <int P, int QB, int KL, int DK>
-__global__ __launch_bounds__(256) void k_knn_tiles_dma(const void *__restrict__ Xh_, const float *__restrict__ cn,
-                                                       int64_t N, int64_t Npad, float beta,
-                                                       int32_t *__restrict__ cand) {
-    using FT = typename Frag<P>::T;
-    constexpr int ESZ = Frag<P>::ESZ;
-    constexpr int DP = 32 * DK;
-    constexpr int KS = (P == PREC_FP32) ? DP / 2 : DP / 16;
-    constexpr int ROWB = DP * ESZ;
-    constexpr int CB = (ROWB >= 512) ? 32 : 64;
-    constexpr int CHUNKS = ROWB / 16;
-    constexpr int NV = CB * ROWB / 16 / 256;
-    constexpr int SB = CB * ROWB;
-    static_assert(NV >= 1 && NV * 4096 == SB, "stage must split into whole wave transfers");
-    static_assert((CHUNKS & (CHUNKS - 1)) == 0, "swizzle needs a power-of-two chunk count");
-
-    __shared__ __align__(16) unsigned char lds0[SB];
-    __shared__ __align__(16) unsigned char lds1[SB];
-    __shared__ __align__(16) unsigned char lds2[SB];
-    __shared__ __align__(16) float ldc0[4][CB];
-    __shared__ __align__(16) float ldc1[4][CB];
-    __shared__ __align__(16) float ldc2[4][CB];
-
-    const unsigned char *Xh = (const unsigned char *)Xh_;
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int h = lane >> 5;
-    const int col = lane & 31;
-    const int64_t q0 = ((int64_t)blockIdx.x * 4 + wave) * (32 * QB);
-
-    FT qf[QB][KS];
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-        const unsigned char *row = Xh + (q0 + qb * 32 + col) * (int64_t)ROWB;
-        if constexpr (P == PREC_FP32) {
-#pragma unroll
-            for (int s = 0; s < KS; s += 4) {
-                float4 v = *(const float4 *)(row + (h * (DP / 2) + s) * 4);
-                qf[qb][s] = v.x;
-                qf[qb][s + 1] = v.y;
-                qf[qb][s + 2] = v.z;
-                qf[qb][s + 3] = v.w;
-            }
-        } else {
-#pragma unroll
-            for (int s = 0; s < KS; ++s) qf[qb][s] = *(const uint4 *)(row + s * 32 + h * 16);
-        }
-    }
-    float ld[QB][KL];
-    int li[QB][KL];
-    float thr[QB];
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-#pragma unroll
-        for (int p = 0; p < KL; ++p) {
-            ld[qb][p] = INFINITY;
-            li[qb][p] = -1;
-        }
-        thr[qb] = INFINITY;
-    }
-
-    const int64_t nstages = Npad / CB;
-    const int64_t last = nstages - 1;
-    const int swz = col & (CHUNKS - 1);  // rows rb*32 + col share col's swizzle (CHUNKS <= 32 divides 32)
-
-#define NCVIS_DMA(STG, LB, LC) \
-    dma_stage<NV, CB, ROWB, CHUNKS>(Xh, cn, (STG) < last ? (STG) : last, LB, LC[wave], wave, lane)
-
-#define NCVIS_DMA_COMPUTE(LB, LC, ST)                                                                  \
-    do {                                                                                               \
-        _Pragma("unroll") for (int rb = 0; rb < CB / 32; ++rb) {                                       \
-            const int64_t c0 = (ST) * CB + rb * 32;                                                    \
-            const unsigned char *arow = LB + (rb * 32 + col) * ROWB;                                   \
-            f32x16_t acc[QB];                                                                          \
-            _Pragma("unroll") for (int qb = 0; qb < QB; ++qb) acc[qb] = f32x16_t{};                    \
-            if constexpr (P == PREC_FP32) {                                                            \
-                _Pragma("unroll") for (int s = 0; s < KS; s += 4) {                                    \
-                    const int ch = (h * (DP / 2) + s) / 4;                                             \
-                    float4 a4 = *(const float4 *)(arow + ((ch ^ swz) * 16));                          \
-                    _Pragma("unroll") for (int qb = 0; qb < QB; ++qb) {                                \
-                        acc[qb] = mfma_step<P>(a4.x, qf[qb][s], acc[qb]);                              \
-                        acc[qb] = mfma_step<P>(a4.y, qf[qb][s + 1], acc[qb]);                          \
-                        acc[qb] = mfma_step<P>(a4.z, qf[qb][s + 2], acc[qb]);                          \
-                        acc[qb] = mfma_step<P>(a4.w, qf[qb][s + 3], acc[qb]);                          \
-                    }                                                                                  \
-                }                                                                                      \
-            } else {                                                                                   \
-                _Pragma("unroll") for (int s = 0; s < KS; ++s) {                                       \
-                    uint4 a = *(const uint4 *)(arow + (((2 * s + h) ^ swz) * 16));                    \
-                    _Pragma("unroll") for (int qb = 0; qb < QB; ++qb) acc[qb] =                        \
-                        mfma_step<P>(a, qf[qb][s], acc[qb]);                                           \
-                }                                                                                      \
-            }                                                                                          \
-            float cnv[16];                                                                             \
-            _Pragma("unroll") for (int g = 0; g < 4; ++g) {                                            \
-                float4 v = *(const float4 *)(&LC[wave][rb * 32 + 8 * g + 4 * h]);                      \
-                cnv[4 * g] = v.x;                                                                      \
-                cnv[4 * g + 1] = v.y;                                                                  \
-                cnv[4 * g + 2] = v.z;                                                                  \
-                cnv[4 * g + 3] = v.w;                                                                  \
-            }                                                                                          \
-            _Pragma("unroll") for (int qb = 0; qb < QB; ++qb) {                                        \
-                float m = INFINITY;                                                                    \
-                _Pragma("unroll") for (int i = 0; i < 16; ++i) m = fminf(m, fmaf(beta, acc[qb][i], cnv[i])); \
-                if (__any(m < thr[qb])) {                                                              \
-                    _Pragma("unroll") for (int i = 0; i < 16; ++i) {                                   \
-                        float sc = fmaf(beta, acc[qb][i], cnv[i]);                                     \
-                        bool pass = sc < thr[qb];                                                      \
-                        if (__any(pass)) {                                                             \
-                            if (pass) {                                                                \
-                                int ci = (int)(c0 + 8 * (i >> 2) + 4 * h + (i & 3));                   \
-                                list_insert<KL>(ld[qb], li[qb], sc, ci);                               \
-                                thr[qb] = ld[qb][KL - 1];                                              \
-                            }                                                                          \
-                        }                                                                              \
-                    }                                                                                  \
-                }                                                                                      \
-            }                                                                                          \
-        }                                                                                              \
-    } while (0)
-
-    // one step: wait for stage ST (only the next stage's transfers may still be in flight),
-    // barrier, start stage ST+2 into the buffer stage ST-1 used, then compute stage ST
-#define NCVIS_DMA_STEP(ST, LB, LC, NLB, NLC)                                                           \
-    do {                                                                                               \
-        NCVIS_WAIT_VMCNT(NV + 1);                                                                      \
-        NCVIS_BARRIER_NOFENCE();                                                                       \
-        NCVIS_DMA((ST) + 2, NLB, NLC);                                                                 \
-        NCVIS_DMA_COMPUTE(LB, LC, ST);                                                                 \
-    } while (0)
-
-    NCVIS_DMA(0, lds0, ldc0);
-    NCVIS_DMA(1, lds1, ldc1);
-    for (int64_t st = 0; st < nstages; st += 3) {
-        NCVIS_DMA_STEP(st, lds0, ldc0, lds2, ldc2);
-        if (st + 1 >= nstages) break;
-        NCVIS_DMA_STEP(st + 1, lds1, ldc1, lds0, ldc0);
-        if (st + 2 >= nstages) break;
-        NCVIS_DMA_STEP(st + 2, lds2, ldc2, lds1, ldc1);
-    }
-    NCVIS_WAIT_VMCNT(0);
-#undef NCVIS_DMA_STEP
-#undef NCVIS_DMA_COMPUTE
-#undef NCVIS_DMA
-
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-        int64_t q = q0 + qb * 32 + col;
-        if (q < N) {
-            int32_t *o = cand + q * (2 * KL) + h * KL;
-#pragma unroll
-            for (int p = 0; p < KL; ++p) o[p] = li[qb][p];
-        }
-    }
 }
 
 // ------------------------------------------------------------------------ exact distances
@@ -1128,142 +372,6 @@ __global__ __launch_bounds__(256) void k_knn_generic(const float *__restrict__ X
     }
 }
 
-// ------------------------------------------------------------------------ host launcher
-template <int P, int QB, int KL>
-static void launch_tiles_dk(int DK, dim3 grid, const void *Xh, const float *cn, int64_t N, int64_t Npad,
-                            float beta, int32_t *cand, hipStream_t st) {
-    const Tuning &tu = tuning();
-    const int select = tu.knn_select >= 0 ? tu.knn_select : 0;
-    const int dma = tu.knn_dma >= 0 ? tu.knn_dma : (P == PREC_FP32 ? 1 : 0);
-    if (select == 1) {
-        DevBuf<uint2> sbuf((size_t)Npad * 2 * 4 * KL);
-        switch (DK) {
-            case 1: k_knn_tiles_buf<P, QB, KL, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, sbuf.p, cand); break;
-            case 2: k_knn_tiles_buf<P, QB, KL, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, sbuf.p, cand); break;
-            case 4: k_knn_tiles_buf<P, QB, KL, 4><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, sbuf.p, cand); break;
-            case 8: k_knn_tiles_buf<P, QB, KL, 8><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, sbuf.p, cand); break;
-            default: throw HipError("[ncvis_amd] unsupported padded dimension");
-        }
-        NCVIS_LAUNCH_CHECK();
-        NCVIS_HIP_CHECK(hipStreamSynchronize(st));  // sbuf is released on return
-        return;
-    }
-    if (dma) {
-        switch (DK) {
-            case 1: k_knn_tiles_dma<P, QB, KL, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            case 2: k_knn_tiles_dma<P, QB, KL, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            case 4: k_knn_tiles_dma<P, QB, KL, 4><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            case 8: k_knn_tiles_dma<P, QB, KL, 8><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            default: throw HipError("[ncvis_amd] unsupported padded dimension");
-        }
-        return;
-    }
-    if constexpr (P != PREC_FP32 && QB == 2 && KL == 16) {
-        if (tu.knn_skew != 0) {
-            if (tu.knn_stage_mult == 2) {
-                switch (DK) {
-                    case 1: k_knn_tiles_sk<P, KL, 1, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-                    case 2: k_knn_tiles_sk<P, KL, 2, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-                    case 4: k_knn_tiles_sk<P, KL, 4, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-                    case 8: k_knn_tiles_sk<P, KL, 8, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-                    default: throw HipError("[ncvis_amd] unsupported padded dimension");
-                }
-            } else {
-                switch (DK) {
-                    case 1: k_knn_tiles_sk<P, KL, 1, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-                    case 2: k_knn_tiles_sk<P, KL, 2, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-                    case 4: k_knn_tiles_sk<P, KL, 4, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-                    case 8: k_knn_tiles_sk<P, KL, 8, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-                    default: throw HipError("[ncvis_amd] unsupported padded dimension");
-                }
-            }
-            return;
-        }
-    }
-    if (tu.knn_stage_mult == 2) {
-        switch (DK) {
-            case 1: k_knn_tiles<P, QB, KL, 1, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            case 2: k_knn_tiles<P, QB, KL, 2, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            case 4: k_knn_tiles<P, QB, KL, 4, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            case 8: k_knn_tiles<P, QB, KL, 8, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            default: throw HipError("[ncvis_amd] unsupported padded dimension");
-        }
-        return;
-    }
-    switch (DK) {
-        case 1: k_knn_tiles<P, QB, KL, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-        case 2: k_knn_tiles<P, QB, KL, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-        case 4: k_knn_tiles<P, QB, KL, 4><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-        case 8: k_knn_tiles<P, QB, KL, 8><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-        default: throw HipError("[ncvis_amd] unsupported padded dimension");
-    }
-}
-
-template <int P>
-static constexpr int qb_for() {
-    return P == PREC_FP32 ? 1 : 2;
-}
-
-template <int P>
-static void launch_tiles(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                         int32_t *cand, hipStream_t st) {
-    constexpr int QB = qb_for<P>();
-    dim3 grid((unsigned)(Npad / (4 * 32 * QB)));
-    if (KL == 16)
-        launch_tiles_dk<P, QB, 16>(DK, grid, Xh, cn, N, Npad, beta, cand, st);
-    else
-        launch_tiles_dk<P, QB, 32>(DK, grid, Xh, cn, N, Npad, beta, cand, st);
-}
-
-// k + 1 in 17..64 (lists of 32 or 64 per lane half): one query set per wave, since two sets'
-// lists do not fit in VGPRs next to the query fragments (16-bit: register-staged tiles; FP32:
-// the LDS-DMA tiles, which are one set per wave anyway).
-template <int P, int KL>
-static void launch_tiles_q1(int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                            int32_t *cand, hipStream_t st) {
-    dim3 grid((unsigned)(Npad / (4 * 32)));
-    if constexpr (P == PREC_FP32) {
-        switch (DK) {
-            case 1: k_knn_tiles_dma<P, 1, KL, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            case 2: k_knn_tiles_dma<P, 1, KL, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            case 4: k_knn_tiles_dma<P, 1, KL, 4><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            case 8: k_knn_tiles_dma<P, 1, KL, 8><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            default: throw HipError("[ncvis_amd] unsupported padded dimension");
-        }
-    } else {
-        switch (DK) {
-            case 1: k_knn_tiles<P, 1, KL, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            case 2: k_knn_tiles<P, 1, KL, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            case 4: k_knn_tiles<P, 1, KL, 4><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            case 8: k_knn_tiles<P, 1, KL, 8><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-            default: throw HipError("[ncvis_amd] unsupported padded dimension");
-        }
-    }
-}
-
-// Rows of 257..512 elements (16-bit GEMM): one query set per wave, so the query fragments
-// (Dpad / 16 per lane) still fit in VGPRs; 32-row stages.
-template <int P, int KL>
-static void launch_tiles_wide_kl(int DK, dim3 grid, const void *Xh, const float *cn, int64_t N, int64_t Npad,
-                                 float beta, int32_t *cand, hipStream_t st) {
-    switch (DK) {
-        case 10: k_knn_tiles<P, 1, KL, 10><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-        case 12: k_knn_tiles<P, 1, KL, 12><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-        case 16: k_knn_tiles<P, 1, KL, 16><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
-        default: throw HipError("[ncvis_amd] unsupported padded dimension");
-    }
-}
-
-template <int P>
-static void launch_tiles_wide(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                              int32_t *cand, hipStream_t st) {
-    dim3 grid((unsigned)(Npad / (4 * 32)));
-    if (KL == 16)
-        launch_tiles_wide_kl<P, 16>(DK, grid, Xh, cn, N, Npad, beta, cand, st);
-    else
-        launch_tiles_wide_kl<P, 32>(DK, grid, Xh, cn, N, Npad, beta, cand, st);
-}
-
 void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32_t *d_inds, float *d_dists,
              hipStream_t st, KnnStats *stats) {
     if (N < 2 || D < 1 || k < 1 || k > N - 1) throw HipError("[ncvis_amd] knn: invalid sizes");
@@ -1291,7 +399,7 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
     if (tiled) {
         const int DK = D <= 32 ? 1 : D <= 64 ? 2 : D <= 128 ? 4 : D <= 256 ? 8 : D <= 320 ? 10 : D <= 384 ? 12 : 16;
         const int Dpad = 32 * DK;
-        const int QB = (wide || KL >= 32) ? 1 : (prec == PREC_FP32) ? qb_for<PREC_FP32>() : qb_for<PREC_BF16>();
+        const int QB = (wide || KL >= 32 || prec == PREC_FP32) ? 1 : 2;
         const int64_t qtile = 4 * 32 * QB;
         const int64_t Npad = ((N + qtile - 1) / qtile) * qtile;
         const int esz = (prec == PREC_FP32) ? 4 : 2;
@@ -1313,29 +421,18 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
         DevBuf<int32_t> cand((size_t)N * 2 * KL);
         const float beta = is_l2 ? -2.0f : -1.0f;
         NCVIS_HIP_CHECK(hipEventRecord(e0, st));
-        if (KL == 64 && !wide) {
-            switch (prec) {
-                case PREC_FP32: launch_tiles_q1<PREC_FP32, 64>(DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
-                case PREC_BF16: launch_tiles_q1<PREC_BF16, 64>(DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
-                default: launch_tiles_q1<PREC_FP16, 64>(DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
-            }
-        } else if (KL == 32 && !wide && prec != PREC_FP32) {
-            if (prec == PREC_BF16)
-                launch_tiles_q1<PREC_BF16, 32>(DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
-            else
-                launch_tiles_q1<PREC_FP16, 32>(DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
-        } else if (wide) {
-            if (prec == PREC_BF16)
-                launch_tiles_wide<PREC_BF16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
-            else
-                launch_tiles_wide<PREC_FP16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
-        } else {
-            switch (prec) {
-                case PREC_FP32: launch_tiles<PREC_FP32>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
-                case PREC_BF16: launch_tiles<PREC_BF16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
-                default: launch_tiles<PREC_FP16>(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st); break;
-            }
-        }
+        if (wide)
+            knn_tiles_wide(prec, KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
+        else if (KL == 16)
+            knn_tiles16(prec, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
+        else if (prec == PREC_FP32 && tuning().knn_dma != 0)
+            knn_tiles_q1_f32dma(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
+        else if (prec == PREC_FP32)
+            knn_tiles_q1_f32(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
+        else if (prec == PREC_BF16)
+            knn_tiles_q1_bf16(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
+        else
+            knn_tiles_q1_fp16(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
         NCVIS_LAUNCH_CHECK();
         NCVIS_HIP_CHECK(hipEventRecord(e1, st));
         const int vec4 = (D % 4 == 0) && ((uintptr_t)Xp % 16 == 0);

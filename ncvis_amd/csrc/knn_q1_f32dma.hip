@@ -1,0 +1,11 @@
+// kNN candidate tiles with lists of 32 or 64 per lane half (k + 1 in 17..64), FP32 GEMM, LDS-DMA stages.
+#include "knn_tiles.h"
+
+namespace ncvis_hip {
+
+void knn_tiles_q1_f32dma(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
+                        int32_t *cand, hipStream_t st) {
+    launch_tiles_q1<PREC_FP32, 1>(KL, DK, N, Npad, Xh, cn, beta, cand, st);
+}
+
+}  // namespace ncvis_hip
