@@ -41,9 +41,13 @@ enum ncvis_status { NCVIS_OK = 0, NCVIS_EINVAL = -1, NCVIS_EHIP = -2, NCVIS_EINT
 typedef struct ncvis_handle ncvis_handle;
 
 /* ncvis::NCVis::NCVis (src/ncvis.hpp:54). n_noise may be NULL (3 per epoch, src/ncvis.cpp:16-20)
- * and is copied. n_threads is the number of sequential SGD chains (the reference's OpenMP
- * threads, each seeded random_seed + thread id); 0 selects a GPU-sized chain count. M and
- * ef_construction are accepted for signature compatibility; the search is exact. */
+ * and is copied; entries must be below 2^32 - 1. n_threads is the reference's OpenMP thread
+ * count. Each GPU SGD chain plays one such thread (its own pcg64(random_seed + chain), a static
+ * chunk of the edges, a private Q copy). By default the chain count is GPU-sized (E / 64, at most
+ * 2^17) whatever n_threads says, because a few sequential chains leave the GPU idle. Option
+ * "exact_threads" = 1 runs exactly n_threads chains instead; n_threads = 1 then reproduces the
+ * reference at one thread. M and ef_construction are accepted for signature compatibility; the
+ * search is exact. */
 int ncvis_create(size_t d, size_t n_threads, size_t n_neighbors, size_t M, size_t ef_construction,
                  size_t random_seed, int n_epochs, int n_init_epochs, float a, float b, float alpha,
                  float alpha_Q, const size_t *n_noise, int dist, ncvis_handle **out);
@@ -55,9 +59,10 @@ const char *ncvis_last_error(const ncvis_handle *h);
 
 /* Options (no reference counterpart):
  *   "knn_precision"   NCVIS_PREC_*  (default NCVIS_PREC_FP16)
- *   "atomic_updates"  1: Hogwild Y updates with float atomics (default for n_threads != 1),
+ *   "atomic_updates"  1: Hogwild Y updates with atomics (default when more than one chain runs),
  *                     0: plain read-modify-write stores like the reference
- *   "chains"          override the SGD chain count (0 = from n_threads) */
+ *   "exact_threads"   1: run exactly n_threads chains (default 0: GPU-sized chain count)
+ *   "chains"          override the SGD chain count (0 = none) */
 int ncvis_set_option(ncvis_handle *h, const char *key, double value);
 
 /* Process-wide tuning knobs (no reference counterpart). None changes a single-chain fit; the kNN
@@ -101,18 +106,23 @@ int ncvis_symmetrize_device(const int32_t *d_inds, const float *d_dists, size_t 
                             int32_t *d_col, float *d_dist, int64_t *n_edges, void *stream);
 
 /* Random layout + n_init_epochs propagation epochs; n_threads = the reference's thread count
- * whose static schedule splits the random draws. */
+ * whose static schedule splits the random draws. The graph is validated first (row_ptr starts
+ * at 0 and never decreases, columns lie in [0, N)); a bad graph returns NCVIS_EINVAL. */
 int ncvis_init_embedding_device(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col, float alpha,
                                 int n_init_epochs, uint64_t seed, int64_t n_threads, float *dY, void *stream);
 
-/* SGD epochs. Q is host in/out; q_trace (host, optional) receives Q after each epoch. */
+/* SGD epochs. Q is host in/out; q_trace (host, optional) receives Q after each epoch.
+ * n_threads = the exact chain count (< 1: GPU-sized). n_noise entries must be below 2^32 - 1.
+ * The graph is validated first (row_ptr [N+1] from 0 to n_edges, never decreasing; columns in
+ * [0, N)); a bad graph returns NCVIS_EINVAL. dY needs only float alignment. */
 int ncvis_optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t n_edges,
                           float *dY, float *Q, int n_epochs, const size_t *n_noise, float a, float b, float alpha,
                           float alpha_Q, uint64_t seed, int64_t n_threads, int atomic_updates, float *q_trace,
                           void *stream);
 
 /* NCE loss over explicit negatives d_neg [E, n] int32. d_loss / d_gradQ: device double;
- * d_gradY: device float [N, d], accumulated into (may be NULL). */
+ * d_gradY: device float [N, d], accumulated into (may be NULL). The graph and the negatives are
+ * validated first (indices in [0, N)); a bad index returns NCVIS_EINVAL. */
 int ncvis_nce_loss_device(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t n_edges,
                           const int32_t *d_neg, int n, const float *dY, float Q, float a, float b, double *d_loss,
                           float *d_gradY, double *d_gradQ, void *stream);

@@ -7,9 +7,10 @@ arguments, defaults, noise plan and ValueErrors (:48-163). The C++ class runs th
 through ``libncvis_amd.so``.
 
 Differences a user can observe:
-  * ``n_threads`` counts SGD chains (the reference's OpenMP threads); ``n_threads < 1`` picks a
-    GPU-sized chain count instead of ``cpu_count()``. ``n_threads=1`` reproduces the reference
-    at one thread.
+  * ``n_threads`` is accepted, but the GPU runs a GPU-sized number of SGD chains (each chain plays
+    one of the reference's OpenMP threads) whatever it says. Extra keyword ``exact_threads=True``
+    runs exactly ``n_threads`` chains; ``n_threads=1, exact_threads=True`` reproduces the
+    reference at one thread.
   * ``M`` / ``ef_construction`` are accepted and ignored: the neighbour search is exact.
   * extra keyword ``knn_precision`` ('fp16' default, 'bf16', 'fp32'): precision of the candidate
     GEMM; final neighbour distances are always exact FP32.
@@ -114,18 +115,20 @@ cdef class NCVisWrapper:
 class NCVis:
     def __init__(self, d=2, n_threads=-1, n_neighbors=15, M=8, ef_construction=100, random_seed=42, n_epochs=50,
                  n_init_epochs=20, spread=1., min_dist=0.4, a=None, b=None, alpha=1., alpha_Q=1., n_noise=None,
-                 distance="euclidean", knn_precision="fp16"):
+                 distance="euclidean", knn_precision="fp16", exact_threads=False):
         """
         Creates new NCVis instance (parameters as the reference's wrapper/ncvis.pyx:49-101).
 
-        d : embedding dimensionality. n_threads : SGD chains (< 1: GPU-sized). n_neighbors :
+        d : embedding dimensionality. n_threads : the reference's thread count; the GPU picks its
+        own chain count unless exact_threads is set (then n_threads chains). n_neighbors :
         neighbours per point. M, ef_construction : accepted for compatibility (exact search).
         random_seed : chain t uses pcg64(random_seed + t). n_epochs / n_init_epochs : optimisation
         and initialisation epochs. spread, min_dist, a, b : kernel 1/(1+a|x-y|^(2b)). alpha,
         alpha_Q : learning rates of the embedding and of the normaliser Q. n_noise : int, 1-D
         ndarray (per-epoch plan) or None (plan ramping up as linspace(0,1,n_epochs)**3). distance :
         'euclidean', 'cosine', 'correlation' or 'inner_product'. knn_precision : 'fp16', 'bf16' or
-        'fp32' candidate GEMM.
+        'fp32' candidate GEMM. exact_threads : run exactly n_threads SGD chains (chain t is the
+        reference's thread t; n_threads=1 is the reference at one thread).
         """
         self.d = d
         negative_plan, n_epochs = negative_plan_for(n_noise, n_epochs)
@@ -155,6 +158,8 @@ class NCVis:
         self.model = NCVisWrapper(d, n_threads, n_neighbors, M, ef_construction, random_seed, n_epochs, n_init_epochs,
                                   a, b, alpha, alpha_Q, negative_plan, distances[distance])
         self.model.set_option("knn_precision", _PRECISIONS[knn_precision])
+        if exact_threads:
+            self.model.set_option("exact_threads", 1)
 
     def fit_transform(self, X):
         """

@@ -24,6 +24,7 @@ struct ncvis_handle {
     int precision = PREC_FP16;
     int atomic_updates = -1;  // -1: atomics unless a single chain
     int64_t chains_override = 0;
+    int exact_threads = 0;  // 1: exactly n_threads chains (chain t = the reference's thread t)
     std::string last_error;
     hipStream_t stream = nullptr;
     // stats of the last fit
@@ -74,9 +75,12 @@ static int64_t auto_chains(int64_t E) {
     return std::max<int64_t>(1, std::min<int64_t>(c, t.max_chains));
 }
 
+// n_threads is the reference's OpenMP thread count. A GPU chain is one such thread, and a few
+// sequential chains would leave the GPU idle (one chain at 1M points is ~8e9 dependent samples),
+// so the chain count is GPU-sized unless the caller asks for exactly n_threads chains.
 static int64_t resolve_chains(const ncvis_handle *h, int64_t E) {
     if (h->chains_override > 0) return h->chains_override;
-    if (h->n_threads > 0) return (int64_t)h->n_threads;
+    if (h->exact_threads && h->n_threads > 0) return (int64_t)h->n_threads;
     return auto_chains(E);
 }
 
@@ -184,8 +188,9 @@ int ncvis_create(size_t d, size_t n_threads, size_t n_neighbors, size_t M, size_
         h->dist = dist;
         h->n_noise.resize((size_t)n_epochs);
         for (int i = 0; i < n_epochs; ++i) h->n_noise[i] = n_noise ? (uint64_t)n_noise[i] : 3;  // ncvis.cpp:16-20
+        // the epoch kernels count n_noise + 1 samples per edge in 32 bits
         for (uint64_t v : h->n_noise)
-            if (v > 0xFFFFFFFFull) {
+            if (v >= 0xFFFFFFFFull) {
                 delete h;
                 throw std::invalid_argument("[ncvis_amd] n_noise entry too large");
             }
@@ -221,6 +226,8 @@ int ncvis_set_option(ncvis_handle *h, const char *key, double value) {
         } else if (k == "chains") {
             if (value < 0) throw std::invalid_argument("[ncvis_amd] chains must be >= 0");
             h->chains_override = (int64_t)value;
+        } else if (k == "exact_threads") {
+            h->exact_threads = value != 0;
         } else {
             throw std::invalid_argument("[ncvis_amd] unknown option '" + k + "'");
         }
@@ -383,6 +390,12 @@ int ncvis_init_embedding_device(size_t N, size_t d, const int64_t *d_row_ptr, co
                                 int n_init_epochs, uint64_t seed, int64_t n_threads, float *dY, void *stream) {
     return guarded(g_last_error, [&] {
         if (!d_row_ptr || !dY) throw std::invalid_argument("[ncvis_amd] init: null pointer");
+        if (N < 1) throw std::invalid_argument("[ncvis_amd] init: N must be >= 1");
+        int64_t E = 0;
+        NCVIS_HIP_CHECK(hipMemcpyAsync(&E, d_row_ptr + N, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                       (hipStream_t)stream));
+        NCVIS_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+        check_graph(d_row_ptr, (int64_t)N, d_col, E, nullptr, 0, (hipStream_t)stream);
         init_embedding_run((int64_t)N, (int)d, d_row_ptr, d_col, alpha, n_init_epochs, seed,
                            n_threads < 1 ? 1 : n_threads, dY, (hipStream_t)stream);
     });
@@ -397,9 +410,11 @@ int ncvis_optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const in
         if (n_epochs > 0 && !n_noise) throw std::invalid_argument("[ncvis_amd] optimize: null n_noise");
         std::vector<uint64_t> nn((size_t)std::max(n_epochs, 0));
         for (int i = 0; i < n_epochs; ++i) {
-            if ((uint64_t)n_noise[i] > 0xFFFFFFFFull) throw std::invalid_argument("[ncvis_amd] n_noise too large");
+            if ((uint64_t)n_noise[i] >= 0xFFFFFFFFull) throw std::invalid_argument("[ncvis_amd] n_noise too large");
             nn[i] = (uint64_t)n_noise[i];
         }
+        if (N < 2) throw std::invalid_argument("[ncvis_amd] optimize: N must be >= 2");
+        check_graph(d_row_ptr, (int64_t)N, d_col, n_edges, nullptr, 0, (hipStream_t)stream);
         SgdParams p;
         p.a = a;
         p.b = b;
@@ -420,6 +435,8 @@ int ncvis_nce_loss_device(size_t N, size_t d, const int64_t *d_row_ptr, const in
     return guarded(g_last_error, [&] {
         if (!d_row_ptr || !d_col || !d_neg || !dY || !d_loss)
             throw std::invalid_argument("[ncvis_amd] nce_loss: null pointer");
+        if (N < 1 || n < 1) throw std::invalid_argument("[ncvis_amd] nce_loss: needs N >= 1 and n >= 1");
+        check_graph(d_row_ptr, (int64_t)N, d_col, n_edges, d_neg, n_edges * (int64_t)n, (hipStream_t)stream);
         nce_loss_run((int64_t)N, (int)d, d_row_ptr, d_col, n_edges, d_neg, n, dY, Q, a, b, d_loss, d_gradY, d_gradQ,
                      (hipStream_t)stream);
     });

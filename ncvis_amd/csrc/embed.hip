@@ -20,7 +20,9 @@
 // NCE loss (the objective the update of ncvis.cpp:304-338 descends):
 //   k_nce_loss     per edge: positive + n explicit negatives; loss and dL/dQ as FP64 partial
 //                  sums, dL/dY scattered with FP32 atomics.
+#include <algorithm>
 #include <cfloat>
+#include <stdexcept>
 #include <vector>
 
 #include "common.h"
@@ -504,7 +506,10 @@ static void launch_hw(int d, dim3 grid, float *Y, const int64_t *row_ptr, const 
 static void launch_epoch(int d, dim3 grid, float *Y, const int64_t *row_ptr, const int32_t *col, int64_t E,
                          int64_t T, const int64_t *chain_row, U128 *rng, const float *Qdev, float *qpart,
                          const SampleConst &c, uint64_t N, int atomic_mode, hipStream_t st) {
-    if (atomic_mode && d >= 1 && d <= 4) {
+    // the many-chain kernel updates coordinate pairs with 64-bit compare-and-swap, which needs
+    // 8-byte aligned rows; a Y that is only 4-byte aligned takes the float-atomic kernel
+    const bool pairs_aligned = (d % 2 != 0) || ((uintptr_t)Y % 8 == 0);
+    if (atomic_mode && d >= 1 && d <= 4 && pairs_aligned) {
         const Tuning &tu = tuning();
         SampleConst cc = c;
         cc.fast_math = tu.sgd_fast_math;
@@ -584,6 +589,40 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
         }
         *epoch_kernel_ms = tot;
     }
+}
+
+// ------------------------------------------------------------------------ graph checks
+// err bit 1: row_ptr[0] != 0 or row_ptr[N] != E; bit 2: a row end below its start; bit 4: a
+// column index outside [0, N); bit 8: a negative index outside [0, N).
+__global__ void k_check_graph(const int64_t *__restrict__ row_ptr, int64_t N, const int32_t *__restrict__ col,
+                              int64_t E, const int32_t *__restrict__ neg, int64_t n_neg, int *__restrict__ err) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 == 0 && (row_ptr[0] != 0 || row_ptr[N] != E)) atomicOr(err, 1);
+    for (int64_t i = t0; i < N; i += stride)
+        if (row_ptr[i + 1] < row_ptr[i]) atomicOr(err, 2);
+    for (int64_t e = t0; e < E; e += stride)
+        if (col[e] < 0 || col[e] >= N) atomicOr(err, 4);
+    for (int64_t e = t0; e < n_neg; e += stride)
+        if (neg[e] < 0 || neg[e] >= N) atomicOr(err, 8);
+}
+
+void check_graph(const int64_t *d_row_ptr, int64_t N, const int32_t *d_col, int64_t E, const int32_t *d_neg,
+                 int64_t n_neg, hipStream_t st) {
+    if (E > 0 && !d_col) throw std::invalid_argument("[ncvis_amd] edges given without a column array");
+    DevBuf<int> err(1);
+    NCVIS_HIP_CHECK(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    const int64_t work = std::max<int64_t>(std::max<int64_t>(N, E), n_neg);
+    const int nb = (int)std::min<int64_t>(4096, std::max<int64_t>(1, cdiv(work, 256)));
+    k_check_graph<<<nb, 256, 0, st>>>(d_row_ptr, N, d_col, E, d_neg, n_neg, err.p);
+    NCVIS_LAUNCH_CHECK();
+    int herr = 0;
+    NCVIS_HIP_CHECK(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    NCVIS_HIP_CHECK(hipStreamSynchronize(st));
+    if (herr & 1) throw std::invalid_argument("[ncvis_amd] row_ptr must start at 0 and end at the edge count");
+    if (herr & 2) throw std::invalid_argument("[ncvis_amd] row_ptr must be non-decreasing");
+    if (herr & 4) throw std::invalid_argument("[ncvis_amd] column index outside [0, N)");
+    if (herr & 8) throw std::invalid_argument("[ncvis_amd] negative-sample index outside [0, N)");
 }
 
 // ------------------------------------------------------------------------ NCE loss

@@ -73,6 +73,12 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
                   float *Q, const SgdParams &p, hipStream_t st, float *q_trace = nullptr,
                   double *epoch_kernel_ms = nullptr);
 
+// Validates a caller-supplied graph before a kernel indexes with it: row_ptr [N+1] starts at 0,
+// ends at E and never decreases; col [E] and neg [n_neg] (may be null when n_neg = 0) lie in
+// [0, N). Throws std::invalid_argument otherwise. Synchronises `st`.
+void check_graph(const int64_t *d_row_ptr, int64_t N, const int32_t *d_col, int64_t E, const int32_t *d_neg,
+                 int64_t n_neg, hipStream_t st);
+
 // NCE loss and adjoint over an explicit negative table neg[E, n] (int32). dY (float, [N,d],
 // accumulated) and dQ / loss (double) are device pointers; dY may be null for loss only.
 void nce_loss_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t E,

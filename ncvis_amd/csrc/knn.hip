@@ -20,6 +20,7 @@
 //   k_knn_rerank  - exact FP32 distances for the 2*KL candidates of each query, rank by
 //                   (distance, label), drop rank 0, write ranks k..1 (farthest first)
 //   k_knn_generic - the remaining shapes: exact FP32 scan (LDS-tiled) with a max-heap per query
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 
@@ -62,15 +63,118 @@ __device__ inline uint16_t f32_to_bf16_rne(float f) {
 }
 __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// Element k of a padded row as stored for MFMA. 16-bit precisions keep the natural order
-// (lane half h of a 32x32x16 step reads k = 16 s + 8 h .. +7, contiguous). FP32 (32x32x2:
-// lane half h reads k = 2 s + h) is stored permuted, position h*(Dpad/2) + s, so a lane's
-// operands are contiguous too. A and B use the same permutation, so dot products are unchanged.
+// ------------------------------------------------------------------------ GEMM input transform
+// The candidate GEMM ranks candidates by ||c||^2 - 2<q,c>, computed on rounded copies of the
+// rows. Two transforms keep that ranking meaningful for any input scale without changing it:
+//   * L2 only: subtract a per-column offset c_j (translation leaves every distance unchanged);
+//     without it, data on a large offset cancels catastrophically in the expansion and loses its
+//     within-cluster resolution in FP16/BF16. c_j is the column mean rounded to a multiple of
+//     2^floor(log2(max_i |x_ij - mean_j|)), a grid coarse enough that integer data stays integer
+//     (exact in every GEMM precision) and fine enough to leave at most half a spread of offset.
+//   * every distance: one global power-of-two scale s putting max |x - c| in (2^13, 2^14], inside
+//     FP16's range (|x| > 65504 would become inf) and above its subnormals. Scaling by a power of
+//     two multiplies every score by s^2 exactly, so the order is unchanged.
+// Non-finite statistics (non-finite inputs) fall back to c = 0, s = 1.
+// The rerank recomputes exact distances on the untransformed rows.
+
+// Per-column partial statistics over a strided range of rows: MODE 0 sums x (FP64); MODE 1 the
+// maximum of |x - mean_j| (mean may be null: max |x|). Block (64, 4), grid (ceil(D / 64), nb).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_colscan(const float *__restrict__ X, int64_t N, int D,
+                                                 const double *__restrict__ mean, double *__restrict__ part) {
+    __shared__ double red[4][64];
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    double acc = 0.0;
+    if (j < D) {
+        const double m = (MODE == 1 && mean) ? mean[j] : 0.0;
+        for (int64_t i = (int64_t)blockIdx.y * 4 + threadIdx.y; i < N; i += (int64_t)gridDim.y * 4) {
+            const double v = (double)X[i * D + j];
+            if (MODE == 0)
+                acc += v;
+            else
+                acc = fmax(acc, fabs(v - m)) + (v - v);  // + (v - v): a NaN input propagates
+        }
+    }
+    red[threadIdx.y][threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.y == 0 && j < D) {
+        double r = red[0][threadIdx.x];
+        for (int t = 1; t < 4; ++t) {
+            const double v = red[t][threadIdx.x];
+            r = (MODE == 0) ? r + v : fmax(r, v) + (v - v);
+        }
+        part[(int64_t)blockIdx.y * D + j] = r;
+    }
+}
+
+// mean[j] = sum of the nb partial sums / N (one thread per column)
+__global__ void k_colmean(const double *__restrict__ part, int nb, int64_t N, int D, double *__restrict__ mean) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= D) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[(int64_t)b * D + j];
+    mean[j] = s / (double)N;
+}
+
+// One block: cs[j] = c_j (0 without centring), cs[D] = s, from the means (null: no centring)
+// and the nb partial maxima of |x - mean_j|.
+__global__ __launch_bounds__(256) void k_pack_params(const double *__restrict__ mean, const double *__restrict__ pdev,
+                                                     int nb, int D, float *__restrict__ cs) {
+    __shared__ double red_m[256];
+    __shared__ int red_bad[256];
+    double M = 0.0;
+    int bad = 0;
+    for (int j = threadIdx.x; j < D; j += 256) {
+        double dev = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            const double v = pdev[(int64_t)b * D + j];
+            dev = fmax(dev, v) + (v - v);
+        }
+        const double mu = mean ? mean[j] : 0.0;
+        double c = 0.0;
+        if (mean) c = (dev > 0.0) ? rint(mu / exp2(floor(log2(dev)))) * exp2(floor(log2(dev))) : mu;
+        const float cf = (float)c;
+        const double m = dev + fabs(mu - (double)cf);
+        if (!isfinite(m)) bad = 1;
+        M = fmax(M, m);
+        cs[j] = cf;
+    }
+    red_m[threadIdx.x] = M;
+    red_bad[threadIdx.x] = bad;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            red_m[threadIdx.x] = fmax(red_m[threadIdx.x], red_m[threadIdx.x + w]);
+            red_bad[threadIdx.x] |= red_bad[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    bad = red_bad[0];
+    M = red_m[0];
+    if (bad)
+        for (int j = threadIdx.x; j < D; j += 256) cs[j] = 0.f;
+    if (threadIdx.x == 0) {
+        float sc = 1.f;
+        if (!bad && M > 0.0) {
+            int e = 14 - (int)ceil(log2(M));
+            e = e < -100 ? -100 : (e > 100 ? 100 : e);
+            sc = (float)exp2((double)e);
+        }
+        cs[D] = sc;
+    }
+}
+
+// Element k of a padded row as stored for MFMA: (x_k - c_k) * s, rounded to the GEMM precision.
+// 16-bit precisions keep the natural order (lane half h of a 32x32x16 step reads
+// k = 16 s + 8 h .. +7, contiguous). FP32 (32x32x2: lane half h reads k = 2 s + h) is stored
+// permuted, position h*(Dpad/2) + s, so a lane's operands are contiguous too. A and B use the
+// same permutation, so dot products are unchanged.
 template <int P>
 __global__ void k_pack(const float *__restrict__ Xp, int64_t N, int64_t Npad, int D, int Dpad, int is_l2,
-                       void *__restrict__ Xh, float *__restrict__ cn) {
+                       const float *__restrict__ cs, void *__restrict__ Xh, float *__restrict__ cn) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Npad) return;
+    const float sc = cs[D];
     float s = 0.f;
     if (P == PREC_FP32) {
         float *o = (float *)Xh + i * Dpad;
@@ -78,14 +182,14 @@ __global__ void k_pack(const float *__restrict__ Xp, int64_t N, int64_t Npad, in
         for (int p = 0; p < Dpad; ++p) {
             int h = p / half, st = p - h * half;
             int kk = 2 * st + h;
-            float v = (i < N && kk < D) ? Xp[i * D + kk] : 0.f;
+            float v = (i < N && kk < D) ? __fmul_rn(__fsub_rn(Xp[i * D + kk], cs[kk]), sc) : 0.f;
             o[p] = v;
             s = __fadd_rn(s, __fmul_rn(v, v));
         }
     } else {
         uint16_t *o = (uint16_t *)Xh + i * Dpad;
         for (int kk = 0; kk < Dpad; ++kk) {
-            float v = (i < N && kk < D) ? Xp[i * D + kk] : 0.f;
+            float v = (i < N && kk < D) ? __fmul_rn(__fsub_rn(Xp[i * D + kk], cs[kk]), sc) : 0.f;
             float r;
             if (P == PREC_BF16) {
                 uint16_t b = f32_to_bf16_rne(v);
@@ -372,6 +476,23 @@ __global__ __launch_bounds__(256) void k_knn_generic(const float *__restrict__ X
     }
 }
 
+// cs [D + 1]: per-column offsets (0 unless centring) and the scale, all on the device.
+static void gemm_input_transform(const float *Xp, int64_t N, int D, int center, float *cs, hipStream_t st) {
+    const int nb = (int)std::min<int64_t>(512, std::max<int64_t>(1, cdiv(N, 256)));
+    const dim3 grid((unsigned)cdiv(D, 64), (unsigned)nb), block(64, 4);
+    DevBuf<double> part((size_t)nb * D);
+    DevBuf<double> mean;
+    if (center) {
+        mean.alloc((size_t)D);
+        k_colscan<0><<<grid, block, 0, st>>>(Xp, N, D, nullptr, part.p);
+        k_colmean<<<cdiv(D, 256), 256, 0, st>>>(part.p, nb, N, D, mean.p);
+        NCVIS_LAUNCH_CHECK();
+    }
+    k_colscan<1><<<grid, block, 0, st>>>(Xp, N, D, mean.p, part.p);
+    k_pack_params<<<1, 256, 0, st>>>(mean.p, part.p, nb, D, cs);
+    NCVIS_LAUNCH_CHECK();
+}
+
 void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32_t *d_inds, float *d_dists,
              hipStream_t st, KnnStats *stats) {
     if (N < 2 || D < 1 || k < 1 || k > N - 1) throw HipError("[ncvis_amd] knn: invalid sizes");
@@ -405,15 +526,17 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
         const int esz = (prec == PREC_FP32) ? 4 : 2;
         DevBuf<unsigned char> xh((size_t)Npad * Dpad * esz);
         DevBuf<float> cn((size_t)Npad);
+        DevBuf<float> cs((size_t)D + 1);
+        gemm_input_transform(Xp, N, D, is_l2, cs.p, st);
         switch (prec) {
             case PREC_FP32:
-                k_pack<PREC_FP32><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, xh.p, cn.p);
+                k_pack<PREC_FP32><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs.p, xh.p, cn.p);
                 break;
             case PREC_BF16:
-                k_pack<PREC_BF16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, xh.p, cn.p);
+                k_pack<PREC_BF16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs.p, xh.p, cn.p);
                 break;
             case PREC_FP16:
-                k_pack<PREC_FP16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, xh.p, cn.p);
+                k_pack<PREC_FP16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs.p, xh.p, cn.p);
                 break;
             default: throw HipError("[ncvis_amd] knn: unknown precision");
         }

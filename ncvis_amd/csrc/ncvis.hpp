@@ -23,10 +23,12 @@ class NCVis {
     /*!
     @brief Same parameters and defaults as the reference constructor (src/ncvis.hpp:54).
 
-    n_threads is the number of sequential SGD chains (each the reference's OpenMP thread: its
-    own pcg64(random_seed + id), a static chunk of the edges and a private Q copy); 0 selects a
-    GPU-sized chain count. M and ef_construction are accepted and ignored: the neighbour search
-    is exact. Throws std::runtime_error on invalid arguments or when no GPU is usable.
+    n_threads is the reference's OpenMP thread count. Each GPU SGD chain plays one such thread
+    (its own pcg64(random_seed + id), a static chunk of the edges and a private Q copy). The
+    chain count is GPU-sized by default; set_option("exact_threads", 1) runs exactly n_threads
+    chains (n_threads = 1 then reproduces the reference at one thread). M and ef_construction
+    are accepted and ignored: the neighbour search is exact. Throws std::runtime_error on
+    invalid arguments or when no GPU is usable.
     */
     NCVis(size_t d = 2, size_t n_threads = 1, size_t n_neighbors = 30, size_t M = 16, size_t ef_construction = 200,
           size_t random_seed = 42, int n_epochs = 50, int n_init_epochs = 20, float a = 1., float b = 1.,
@@ -42,7 +44,8 @@ class NCVis {
     */
     void fit_transform(const float *const X, size_t N, size_t D, float *Y);
 
-    /*! @brief Library options (see ncvis_set_option): "knn_precision", "atomic_updates", "chains". */
+    /*! @brief Library options (see ncvis_set_option): "knn_precision", "atomic_updates",
+        "exact_threads", "chains". */
     void set_option(const std::string &key, double value);
 
     /*! @brief Stage times of the last fit in ms: knn, symmetrize, init, optimize, total, knn tiles,

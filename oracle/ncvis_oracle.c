@@ -38,9 +38,14 @@
  *   orc_sgd_sample: float64 central finite differences of the NCE loss (orc_nce_loss),
  *     an independent softplus formulation.
  *   orc_optimize: closed-form decay of a single edge with n_noise = 0 (step schedule),
- *     and the sample body above.
- *   Q averaging over T > 1 chains (ncvis.cpp:342-349): parity unpinned -- the
- *     reference is itself nondeterministic for T > 1; only T = 1 is pinned.
+ *     and the sample body above. T > 1 (chain partition, per-thread pcg64(seed + t), the
+ *     Q average of ncvis.cpp:342-349): with alpha = 0 the steps vanish and every chain is
+ *     deterministic, so its Q trace is pinned against an independent restatement of the
+ *     chunking, seeding and averaging in tests/test_oracle.py, against the mean of
+ *     one-thread runs on the single chunks, and against orc_optimize_chains. The T > 1 run
+ *     with alpha != 0 is Hogwild and nondeterministic in the reference itself.
+ *   orc_optimize_chains: equal to orc_optimize at T = 1; at alpha = 0 equal to
+ *     orc_optimize at T threads and to the restatement above.
  */
 #include <math.h>
 #include <omp.h>
@@ -205,42 +210,6 @@ void orc_knn_bruteforce(const float *Xp, int64_t N, int64_t D, int64_t k, int di
                 int64_t src = cnt - 1 - j;
                 inds[q * k + j] = (src >= 1) ? hi[src] : -1;
                 dists[q * k + j] = (src >= 1) ? hd[src] : INFINITY;
-            }
-        }
-        free(hd);
-        free(hi);
-    }
-}
-
-/* The same exact search for a subset of query rows q_idx[0..nq) (timing samples). */
-void orc_knn_bruteforce_queries(const float *Xp, int64_t N, int64_t D, int64_t k, int dist, int n_threads,
-                                const int64_t *q_idx, int64_t nq, int64_t *inds, float *dists) {
-    int64_t kk = k + 1;
-    if (n_threads > 0) omp_set_num_threads(n_threads);
-#pragma omp parallel
-    {
-        float *hd = (float *)malloc(sizeof(float) * kk);
-        int64_t *hi = (int64_t *)malloc(sizeof(int64_t) * kk);
-#pragma omp for schedule(dynamic, 4)
-        for (int64_t r = 0; r < nq; ++r) {
-            int64_t cnt = 0;
-            const float *xq = Xp + q_idx[r] * D;
-            for (int64_t c = 0; c < N; ++c) {
-                float dv = pair_dist(xq, Xp + c * D, D, dist);
-                if (cnt == kk && !pair_less(dv, c, hd[kk - 1], hi[kk - 1])) continue;
-                int64_t p = (cnt < kk) ? cnt++ : kk - 1;
-                while (p > 0 && pair_less(dv, c, hd[p - 1], hi[p - 1])) {
-                    hd[p] = hd[p - 1];
-                    hi[p] = hi[p - 1];
-                    --p;
-                }
-                hd[p] = dv;
-                hi[p] = c;
-            }
-            for (int64_t j = 0; j < k; ++j) {
-                int64_t src = cnt - 1 - j;
-                inds[r * k + j] = (src >= 1) ? hi[src] : -1;
-                dists[r * k + j] = (src >= 1) ? hd[src] : INFINITY;
             }
         }
         free(hd);
@@ -573,10 +542,11 @@ float orc_sgd_sample(float *Y, int64_t d, int64_t id, int64_t other, int is_posi
     return sgd_sample(Y, d, id, other, is_positive, cur_noise, Q, step, p);
 }
 
-/* ncvis.cpp:275-352 with n_threads OpenMP threads (Hogwild for n_threads > 1). */
+/* ncvis.cpp:275-352 with n_threads OpenMP threads (Hogwild for n_threads > 1). q_trace (may be
+ * NULL) receives Q after each epoch. */
 void orc_optimize(int64_t N, int64_t d, int64_t E, const int64_t *src, const int64_t *dst, float *Y, float *Q,
                   int n_epochs, const uint64_t *n_noise, float a, float b, float alpha, float alpha_Q, uint64_t seed,
-                  int n_threads) {
+                  int n_threads, float *q_trace) {
     float Q_cum = 0.;
     float Qs = *Q;
     sgd_params p = {a, b, alpha_Q};
@@ -613,9 +583,57 @@ void orc_optimize(int64_t N, int64_t d, int64_t E, const int64_t *src, const int
             {
                 Qs = Q_cum / nt;
                 Q_cum = 0;
+                if (q_trace) q_trace[epoch] = Qs;
             }
         }
     }
+    *Q = Qs;
+}
+
+/* The same epochs with T chains run one after another inside each epoch (chain 0's whole static
+ * chunk, then chain 1's, ...): chain t is OpenMP thread t of ncvis.cpp:279-349 -- its own
+ * pcg64(seed + t) kept across epochs (:281-283), the libgomp static chunk of the edge loop
+ * (:290-291), a private Q copy started from the epoch's Q (:288) -- and Q becomes the mean of the
+ * T copies (:342-349), summed in chain order. This is one serialisation of the reference's T-thread
+ * Hogwild run; with alpha = 0 every step is 0, Y never changes and each chain's Q copy no longer
+ * depends on the others, so the T-thread run and this one agree up to the order of the float sum
+ * of the T copies. q_trace (may be NULL) receives Q after each epoch. */
+void orc_optimize_chains(int64_t N, int64_t d, int64_t E, const int64_t *src, const int64_t *dst, float *Y,
+                         float *Q, int n_epochs, const uint64_t *n_noise, float a, float b, float alpha,
+                         float alpha_Q, uint64_t seed, int64_t T, float *q_trace) {
+    if (T < 1) T = 1;
+    sgd_params p = {a, b, alpha_Q};
+    pcg64_t *g = (pcg64_t *)malloc(sizeof(pcg64_t) * (size_t)T);
+    for (int64_t t = 0; t < T; ++t) pcg64_seed(&g[t], seed + (uint64_t)t);
+    float Qs = *Q;
+    for (int epoch = 0; epoch < n_epochs; ++epoch) {
+        float step = alpha * (1 - (((float)epoch) / n_epochs) * (((float)epoch) / n_epochs));
+        uint64_t cur_noise = n_noise[epoch];
+        float Q_cum = 0.;
+        for (int64_t t = 0; t < T; ++t) {
+            int64_t s0, e0;
+            orc_static_chunk(E, T, t, &s0, &e0);
+            float Q_copy = Qs;
+            for (int64_t e = s0; e < e0; ++e) {
+                int64_t id = src[e];
+                for (uint64_t j = 0; j < cur_noise + 1; ++j) {
+                    int64_t other;
+                    if (j == 0) {
+                        other = dst[e];
+                    } else {
+                        do {
+                            other = (int64_t)uniform_index(&g[t], (uint64_t)N);
+                        } while (other == id);
+                    }
+                    Q_copy = sgd_sample(Y, d, id, other, j == 0, cur_noise, Q_copy, step, p);
+                }
+            }
+            Q_cum += Q_copy;
+        }
+        Qs = Q_cum / T;
+        if (q_trace) q_trace[epoch] = Qs;
+    }
+    free(g);
     *Q = Qs;
 }
 
@@ -703,7 +721,7 @@ int orc_fit_transform(const float *X, int64_t N, int64_t D, int64_t d, int n_thr
     float init_alpha = 1. / k;
     orc_init_embedding(N, d, E, src, col, init_alpha, n_init_epochs, seed, n_threads, Y);
     double t4 = omp_get_wtime();
-    orc_optimize(N, d, E, src, col, Y, &Q, n_epochs, n_noise, a, b, alpha, alpha_Q, seed, n_threads);
+    orc_optimize(N, d, E, src, col, Y, &Q, n_epochs, n_noise, a, b, alpha, alpha_Q, seed, n_threads, NULL);
     double t5 = omp_get_wtime();
     if (Q_out) *Q_out = Q;
     if (times) {

@@ -54,8 +54,6 @@ def lib():
         L.orc_preprocess.argtypes = [_f32, C.c_int64, C.c_int64, C.c_int, _f32]
         L.orc_knn_bruteforce.argtypes = [_f32, C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_int,
                                          _i64, _f32]
-        L.orc_knn_bruteforce_queries.argtypes = [_f32, C.c_int64, C.c_int64, C.c_int64, C.c_int,
-                                                 C.c_int, _i64, C.c_int64, _i64, _f32]
         L.orc_intro_sort_idx.argtypes = [_i64, C.c_int64, _f32]
         L.orc_symmetrize.argtypes = [_i64, _f32, C.c_int64, C.c_int64, _i64, _i64, _f32]
         L.orc_symmetrize.restype = C.c_int64
@@ -66,7 +64,10 @@ def lib():
         L.orc_sgd_sample.restype = C.c_float
         L.orc_optimize.argtypes = [C.c_int64, C.c_int64, C.c_int64, _i64, _i64, _f32,
                                    C.POINTER(C.c_float), C.c_int, _u64, C.c_float, C.c_float,
-                                   C.c_float, C.c_float, C.c_uint64, C.c_int]
+                                   C.c_float, C.c_float, C.c_uint64, C.c_int, C.c_void_p]
+        L.orc_optimize_chains.argtypes = [C.c_int64, C.c_int64, C.c_int64, _i64, _i64, _f32,
+                                          C.POINTER(C.c_float), C.c_int, _u64, C.c_float, C.c_float,
+                                          C.c_float, C.c_float, C.c_uint64, C.c_int64, C.c_void_p]
         L.orc_nce_loss.argtypes = [C.c_int64, C.c_int64, C.c_int64, _i64, _i64, _i64, C.c_int64,
                                    _f32, C.c_double, C.c_double, C.c_double, C.c_void_p,
                                    C.c_void_p]
@@ -139,19 +140,6 @@ def knn_bruteforce(Xp: np.ndarray, k: int, distance: str = "euclidean", n_thread
     inds = np.empty((N, k), np.int64)
     dists = np.empty((N, k), np.float32)
     lib().orc_knn_bruteforce(Xp, N, D, k, DIST_CODES[distance], n_threads, inds, dists)
-    return inds, dists
-
-
-def knn_bruteforce_queries(Xp: np.ndarray, k: int, queries, distance: str = "euclidean",
-                           n_threads: int = 0):
-    """knn_bruteforce restricted to the query rows ``queries`` (timing samples)."""
-    Xp = np.ascontiguousarray(Xp, np.float32)
-    N, D = Xp.shape
-    q = np.ascontiguousarray(queries, np.int64)
-    inds = np.empty((len(q), k), np.int64)
-    dists = np.empty((len(q), k), np.float32)
-    lib().orc_knn_bruteforce_queries(Xp, N, D, k, DIST_CODES[distance], n_threads, q, len(q), inds,
-                                     dists)
     return inds, dists
 
 
@@ -247,15 +235,33 @@ def init_embedding(N: int, d: int, row_ptr, col, alpha: float, n_init_epochs: in
 
 
 def optimize(Y: np.ndarray, Q: float, row_ptr, col, n_noise, a: float, b: float, alpha: float,
-             alpha_Q: float, seed: int, n_threads: int = 1):
+             alpha_Q: float, seed: int, n_threads: int = 1, trace: bool = False):
+    """ncvis.cpp:275-352 on n_threads OpenMP threads. Returns (Y, Q) or, with trace, (Y, Q, Q per
+    epoch)."""
     src, dst = edges_from_csr(row_ptr, col)
     Y = np.ascontiguousarray(Y, np.float32).copy()
     N, d = Y.shape
     nn = np.ascontiguousarray(n_noise, np.uint64)
     q = C.c_float(Q)
+    qt = np.zeros(max(len(nn), 1), np.float32)
     lib().orc_optimize(N, d, len(src), src, dst, Y, C.byref(q), len(nn), nn, a, b, alpha, alpha_Q,
-                       seed, n_threads)
-    return Y, q.value
+                       seed, n_threads, qt.ctypes.data)
+    return (Y, q.value, qt[:len(nn)]) if trace else (Y, q.value)
+
+
+def optimize_chains(Y: np.ndarray, Q: float, row_ptr, col, n_noise, a: float, b: float,
+                    alpha: float, alpha_Q: float, seed: int, chains: int):
+    """The epochs with `chains` chains run one after another inside each epoch (chain t = the
+    reference's thread t). Returns (Y, Q, Q per epoch)."""
+    src, dst = edges_from_csr(row_ptr, col)
+    Y = np.ascontiguousarray(Y, np.float32).copy()
+    N, d = Y.shape
+    nn = np.ascontiguousarray(n_noise, np.uint64)
+    q = C.c_float(Q)
+    qt = np.zeros(max(len(nn), 1), np.float32)
+    lib().orc_optimize_chains(N, d, len(src), src, dst, Y, C.byref(q), len(nn), nn, a, b, alpha,
+                              alpha_Q, seed, chains, qt.ctypes.data)
+    return Y, q.value, qt[:len(nn)]
 
 
 def sgd_sample(Y: np.ndarray, i: int, o: int, positive: bool, cur_noise: int, Q: float,

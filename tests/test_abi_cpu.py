@@ -95,3 +95,35 @@ def test_one_hip_runtime_whatever_the_import_order():
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip().splitlines()[-1] == "1"
+
+
+def test_create_rejects_n_noise_that_overflows_the_sample_count():
+    # the epoch kernels count n_noise + 1 samples per edge in 32 bits; 2^32 - 1 must be refused
+    # before any GPU work (the check precedes the stream creation, so it runs without a GPU)
+    from ncvis_amd import abi
+    L = abi.lib()
+    h = ctypes.c_void_p()
+    for bad in (0xFFFFFFFF, 1 << 40):
+        plan = (ctypes.c_size_t * 2)(3, bad)
+        rc = L.ncvis_create(2, 1, 15, 16, 200, 42, 2, 20, 1.0, 1.0, 1.0, 1.0, plan, 0, ctypes.byref(h))
+        assert rc == -1, rc  # NCVIS_EINVAL
+        assert b"n_noise" in L.ncvis_get_last_error()
+        assert not h.value
+
+
+def test_ncvis_module_exposes_the_reference_names():
+    # the reference's extension is the module `ncvis` (setup.py:133-134; recipe/test.py:2)
+    import ncvis
+    import ncvis_amd
+    assert ncvis.NCVis is ncvis_amd.NCVis
+    assert ncvis.find_ab_params is ncvis_amd.find_ab_params
+    assert ncvis.NCVisWrapper is ncvis_amd.NCVisWrapper
+
+
+def test_exact_threads_is_a_keyword():
+    # exact_threads selects n_threads chains; it is validated like the reference's kwargs
+    import inspect
+    import ncvis_amd
+    sig = inspect.signature(ncvis_amd.NCVis.__init__)
+    assert sig.parameters["exact_threads"].default is False
+    assert sig.parameters["n_threads"].default == -1

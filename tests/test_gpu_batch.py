@@ -33,6 +33,7 @@ def test_many_equals_one_by_one_single_chain(torch_cuda, n_streams):
     print(f"[case] batch single-chain n_streams={n_streams}", flush=True)
     sets = _sets([(500, 32, 4), (1200, 64, 6), (2500, 16, 8), (800, 100, 3), (1500, 48, 5)])
     h = abi.Handle(d=2, n_threads=1, n_neighbors=15, n_epochs=12, n_init_epochs=5, a=0.77, b=1.22)
+    h.set_option("exact_threads", 1)
     dX = [torch.from_numpy(X).cuda() for X in sets]
     ref = []
     for x in dX:

@@ -198,6 +198,38 @@ def test_knn_recall_float_data(torch_cuda, precision):
     assert recall >= 0.99, recall
 
 
+@pytest.mark.parametrize("transform", ["offset1e3", "offset1e4", "scale1e5", "scale1e-6"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+def test_knn_recall_any_offset_and_scale(torch_cuda, precision, transform):
+    # the candidate GEMM works on centred, power-of-two scaled copies of the rows, so data far
+    # from the origin (cancellation in ||c||^2 - 2<q,c>, lost FP16/BF16 resolution), beyond FP16's
+    # range (|x| > 65504) or below its normal range keeps the exact search's recall
+    _say(f"knn {transform} {precision}")
+    X, _ = synthetic.anisotropic_mixture(N=3000, D=128, n_clusters=10, seed=2)
+    X = X.astype(np.float64)
+    X = {"offset1e3": X + 1e3, "offset1e4": X + 1e4, "scale1e5": X * 1e5, "scale1e-6": X * 1e-6}[transform]
+    X = X.astype(np.float32)
+    gi, gd = gpu_knn(torch_cuda, X, 15, "euclidean", precision)
+    q = np.arange(0, 3000, 5)
+    oi, _ = oracle.knn_fp64(X, 15, "euclidean", queries=q)
+    recall = np.mean([len(set(a) & set(b)) / 15 for a, b in zip(gi[q], oi)])
+    assert recall >= 0.99, recall
+    assert np.all(np.isfinite(gd)) and np.all(np.diff(gd, axis=1) <= 0)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_knn_inner_product_beyond_fp16_range(torch_cuda, precision):
+    # inner_product ranks by <x, y>, which translation would change: the rows are only scaled
+    _say(f"knn inner_product x1e5 {precision}")
+    X, _ = synthetic.anisotropic_mixture(N=2000, D=64, n_clusters=8, seed=4)
+    X = (X.astype(np.float64) * 1e5).astype(np.float32)
+    gi, _ = gpu_knn(torch_cuda, X, 15, "inner_product", precision)
+    q = np.arange(0, 2000, 5)
+    oi, _ = oracle.knn_fp64(X, 15, "inner_product", queries=q)
+    recall = np.mean([len(set(a) & set(b)) / 15 for a, b in zip(gi[q], oi)])
+    assert recall >= 0.99, recall
+
+
 # ------------------------------------------------------------------------------ symmetrize
 @pytest.mark.parametrize("name", ["sym_digits1500.npz", "sym_ties_hubs.npz", "sym_unsorted.npz",
                                   "sym_tiny.npz"])
@@ -269,19 +301,43 @@ def test_optimize_single_chain_d1_and_d3(torch_cuda):
         np.testing.assert_allclose(gy, oy, atol=1e-4, rtol=0)
 
 
-def test_optimize_chains_statistics(torch_cuda):
-    # many chains (Hogwild): not bitwise comparable, but Q must settle like the reference's
-    _say("optimize chains statistics")
+@pytest.mark.parametrize("T", [3, 64, 4096])
+@pytest.mark.parametrize("atomic", [0, 1])
+def test_optimize_chains_alpha0_q_trace(torch_cuda, T, atomic):
+    # alpha = 0: every step is 0 (ncvis.cpp:287), Y never changes and each chain's Q copy depends
+    # only on its static edge chunk, its pcg64(seed + t) stream (kept across epochs) and the
+    # epoch's Q, so the chain partition, the seeding and the per-epoch Q average (:342-349) are
+    # deterministic. atomic = 1 at d = 2 is the default many-chain epoch kernel.
+    _say(f"optimize alpha=0 T={T} atomic={atomic}")
+    X, inds, dists, row_ptr, col = _digits_graph(N=2000, D=32, k=15)
+    N, k = inds.shape
+    Y0 = oracle.init_embedding(N, 2, row_ptr, col, np.float32(1.0 / k), 20, 42, 1)
+    plan = [3, 1, 5, 2]
+    gy, gq, gtrace = gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 0.0, 1.0,
+                                  42, T, atomic)
+    _, oq, otrace = oracle.optimize_chains(Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 0.0, 1.0, 42, T)
+    assert np.array_equal(gy, Y0)
+    print(f"T={T} atomic={atomic} gpu trace {gtrace} oracle {otrace}", flush=True)
+    np.testing.assert_allclose(gtrace, otrace, rtol=1e-5, atol=1e-5)
+
+
+def test_optimize_chains_hogwild_q_within_reference_scatter(torch_cuda):
+    # alpha = 1, many chains: Hogwild is nondeterministic in the reference too; the GPU's final Q
+    # must lie within the scatter of the oracle's T-thread runs over seeds.
+    _say("optimize hogwild Q vs oracle seeds")
     X, inds, dists, row_ptr, col = _digits_graph(N=2000, D=32, k=15)
     N, k = inds.shape
     Y0 = oracle.init_embedding(N, 2, row_ptr, col, np.float32(1.0 / k), 20, 42, 1)
     plan = [3] * 10
-    for atomic in (0, 1):
-        gy, gq, trace = gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 1.0, 1.0,
-                                     42, 64, atomic)
-        oy, oq = oracle.optimize(Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 1.0, 1.0, 42, 4)
+    oq = np.array([oracle.optimize(Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 1.0, 1.0, s, 8)[1]
+                   for s in (42, 43, 44, 45)])
+    for atomic, T in ((0, 64), (1, 64), (1, 0)):
+        gy, gq, _ = gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 1.0, 1.0,
+                                 42, T, atomic)
+        print(f"atomic={atomic} T={T}: gpu Q {gq:.4f}; oracle Q over seeds {oq}", flush=True)
         assert np.all(np.isfinite(gy))
-        assert abs(gq - oq) < 0.5 + 0.1 * abs(oq), (gq, oq)
+        tol = 3 * max(float(oq.std()), 0.01 * abs(float(oq.mean())))
+        assert abs(gq - oq.mean()) <= tol, (gq, oq)
 
 
 # ------------------------------------------------------------------------------ NCE loss
@@ -321,7 +377,7 @@ def test_fit_transform_single_chain_matches_reference(torch_cuda):
     X, _ = synthetic.digits_like(N=800, D=32, seed=9)
     plan = np.array([1, 2], np.uint64)
     vis = ncvis_amd.NCVis(n_threads=1, n_neighbors=15, n_noise=plan, n_init_epochs=20,
-                          knn_precision="fp32", random_seed=42)
+                          knn_precision="fp32", random_seed=42, exact_threads=True)
     Y = vis.fit_transform(X)
     Yo, _, _ = oracle.fit_transform(X, d=2, n_threads=1, n_neighbors=15, random_seed=42,
                                  n_init_epochs=20, a=vis.a, b=vis.b, n_noise=plan)

@@ -1,7 +1,8 @@
-"""The reference's own tests (recipe/test.py) run against ncvis_amd on the GPU, plus the
-full-run parity criteria for the many-chain (Hogwild) mode against the oracle at T threads:
-15-NN preservation within 0.01 and final NCE loss within 2%."""
+"""The reference's own tests (recipe/test.py) run through ``import ncvis`` on the GPU with the
+same calls, plus the full-run parity criteria for the default many-chain mode against the oracle
+run with OpenMP threads: 15-NN preservation within 0.01 and final NCE loss within 2%."""
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -22,20 +23,44 @@ def ncvis():
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    import ncvis_amd
-    return ncvis_amd
+    import ncvis as module
+    return module
+
+
+_DISTANCES = ["euclidean", "cosine", "correlation", "inner_product"]
 
 
 def test_distances(ncvis):
-    # recipe/test.py:7-30 -- 5x3 uniform data, all four distances, k clamped to N-1
+    # recipe/test.py:7-30: 5 x 3 uniform points, every distance, n_neighbors clamped to N - 1
     np.random.seed(42)
     X = np.random.random((5, 3))
-    for distance in ["euclidean", "cosine", "correlation", "inner_product"]:
+    for distance in _DISTANCES:
         _say(f"test_distances {distance}")
         vis = ncvis.NCVis(n_neighbors=15, M=16, ef_construction=200, random_seed=42, n_init_epochs=20,
                           n_epochs=50, min_dist=0.4, n_threads=-1, distance=distance)
         Y = vis.fit_transform(X)
         assert np.all(np.isfinite(Y)), "All entries must be finite"
+
+
+@pytest.mark.timeout(300)
+def test_parallel(ncvis):
+    # recipe/test.py:33-62: 5,000 x 10 uniform points, n_threads 1 then 2, and the reference's
+    # gate on the time ratio t1 / (2 t2) > 0.3
+    np.random.seed(42)
+    X = np.random.random((5 * 10 ** 3, 10))
+    for distance in _DISTANCES:
+        times = {}
+        for n_th in (1, 2):
+            _say(f"test_parallel {distance} n_threads={n_th}")
+            vis = ncvis.NCVis(n_neighbors=15, M=16, ef_construction=200, random_seed=42, n_init_epochs=20,
+                              n_epochs=50, min_dist=0.4, n_threads=n_th, distance=distance)
+            t0 = time.time()
+            Y = vis.fit_transform(X)
+            times[n_th] = time.time() - t0
+            assert np.all(np.isfinite(Y))
+        eff = times[1] / (times[2] * 2)
+        print(f"{distance}: times {times}, efficiency {eff:.2f}", flush=True)
+        assert eff > 0.3, "Parallelization efficiency is too low"
 
 
 def _two_gaussians():
@@ -45,44 +70,38 @@ def _two_gaussians():
     return X, n
 
 
-@pytest.mark.parametrize("n_threads", [-1, 1])
-def test_1d_clustering(ncvis, n_threads):
-    # recipe/test.py:65-86
-    _say(f"test_1d_clustering n_threads={n_threads}")
+@pytest.mark.parametrize("exact", [False, True])
+def test_1d_clustering(ncvis, exact):
+    # recipe/test.py:65-86 (exact=True: one chain, i.e. the reference at one thread)
+    _say(f"test_1d_clustering exact={exact}")
     X, n = _two_gaussians()
+    kw = dict(n_threads=1, exact_threads=True) if exact else dict(n_threads=-1)
     vis = ncvis.NCVis(n_neighbors=15, M=16, ef_construction=200, d=1, n_init_epochs=20, n_epochs=50,
-                      min_dist=0.4, n_threads=n_threads, distance="euclidean", random_seed=42)
+                      min_dist=0.4, distance="euclidean", random_seed=42, **kw)
     Y = vis.fit_transform(X).ravel()
     n_pos = np.count_nonzero(Y - Y.mean() > 0)
     assert np.abs(n_pos - n) < 5, "Clustering quality is too poor"
 
 
-@pytest.mark.parametrize("n_threads", [-1, 1])
-def test_1d_tsne(ncvis, n_threads):
+@pytest.mark.parametrize("exact", [False, True])
+def test_1d_tsne(ncvis, exact):
     # recipe/test.py:89-111
-    _say(f"test_1d_tsne n_threads={n_threads}")
+    _say(f"test_1d_tsne exact={exact}")
     X, n = _two_gaussians()
+    kw = dict(n_threads=1, exact_threads=True) if exact else dict(n_threads=-1)
     vis = ncvis.NCVis(n_neighbors=15, M=16, ef_construction=200, d=1, n_init_epochs=20, n_epochs=50,
-                      a=1, b=1, n_threads=n_threads, distance="euclidean", random_seed=42)
+                      a=1, b=1, distance="euclidean", random_seed=42, **kw)
     Y = vis.fit_transform(X).ravel()
     n_pos = np.count_nonzero(Y - Y.mean() > 0)
     assert np.abs(n_pos - n) < 5, "Clustering quality is too poor"
 
 
-def test_parallel_chains_run(ncvis):
-    # recipe/test.py:33-62 gates CPU thread efficiency; on the GPU the equivalent property is that
-    # the many-chain mode completes every distance on the same data (timing gates are host
-    # dependent and not meaningful for a GPU)
-    np.random.seed(42)
-    X = np.random.random((5 * 10 ** 3, 10))
-    for distance in ["euclidean", "cosine", "correlation", "inner_product"]:
-        _say(f"test_parallel {distance}")
-        vis = ncvis.NCVis(n_neighbors=15, M=16, ef_construction=200, random_seed=42, n_init_epochs=20,
-                          n_epochs=50, min_dist=0.4, n_threads=-1, distance=distance)
-        Y = vis.fit_transform(X)
-        assert np.all(np.isfinite(Y))
+def test_module_names(ncvis):
+    import ncvis_amd
+    assert ncvis.NCVis is ncvis_amd.NCVis and ncvis.find_ab_params is ncvis_amd.find_ab_params
 
 
+# ------------------------------------------------------------------------- full-run parity
 def knn_preservation(X, Y, k=15, queries=None):
     """Mean fraction of each point's exact k nearest neighbours in X that are also among its k
     nearest neighbours in the embedding Y."""
@@ -91,19 +110,17 @@ def knn_preservation(X, Y, k=15, queries=None):
     return float(np.mean([len(set(a) & set(b)) / k for a, b in zip(hx, hy)]))
 
 
-def _loss_setup(X, k, seed=5):
+def _loss_setup(X, k, n_neg, seed=5):
     inds, dists = oracle.knn_bruteforce(oracle.preprocess(X), k)
     row_ptr, col, _ = oracle.symmetrize(inds, dists)
     src, dst = oracle.edges_from_csr(row_ptr, col)
-    neg = np.random.default_rng(seed).integers(0, len(X), (len(src), 5))
+    neg = np.random.default_rng(seed).integers(0, len(X), (len(src), n_neg))
     return src, dst, neg
 
 
 def profile_loss(Y, src, dst, neg, a, b):
     """NCE loss of the embedding Y minimised over the normaliser Q (convex in Q: bisection on
-    dL/dQ). The reference's loss at its own final Q scatters by about +-6% between seeds and
-    thread counts (Q keeps a constant step alpha_Q), while this profile loss agrees to +-0.2%,
-    so it is the loss the parity criterion compares."""
+    dL/dQ)."""
     lo, hi = -30.0, 10.0
     for _ in range(60):
         mid = 0.5 * (lo + hi)
@@ -117,24 +134,32 @@ def profile_loss(Y, src, dst, neg, a, b):
 
 
 def test_full_run_parity_many_chains(ncvis):
-    """digits-shaped data (N=5,620, D=64, integers): GPU many-chain fit vs the oracle run with
-    8 OpenMP threads, default plan, k=15. Criteria: 15-NN preservation within 0.01, final NCE
-    (profile) loss within 2%."""
+    """digits-shaped data (N=5,620, D=64, integers, config 1's shape): the default GPU fit vs
+    the oracle run with 8 OpenMP threads (default plan, k=15) at seeds 42, 43, 44. Criteria:
+    15-NN preservation within 0.01 of the oracle's seed band, and the NCE loss at each run's
+    own final Q (negatives: a fixed table of the last epoch's count per edge, the count that
+    final Q is learned against) within 2% of the oracle's seed band; the loss minimised over Q
+    is held to the same bound."""
     _say("full-run parity digits 5620x64")
     X, _ = synthetic.digits_like(seed=0)
     vis = ncvis.NCVis(n_threads=-1, random_seed=42)
     Y = vis.fit_transform(X)
     Qg = vis.stats()["Q"]
     plan = vis.negative_plan
-    Yo, _, Qo = oracle.fit_transform(X, d=2, n_threads=8, n_neighbors=15, random_seed=42,
-                                     n_init_epochs=20, a=vis.a, b=vis.b, n_noise=plan)
+    src, dst, neg = _loss_setup(X, 15, int(plan[-1]))
     q = np.arange(0, len(X), 3)
     pg = knn_preservation(X, Y, queries=q)
-    po = knn_preservation(X, Yo, queries=q)
-    src, dst, neg = _loss_setup(X, 15)
-    Lg, qg = profile_loss(Y, src, dst, neg, vis.a, vis.b)
-    Lo, qo = profile_loss(Yo, src, dst, neg, vis.a, vis.b)
-    print(f"preservation gpu={pg:.4f} oracle={po:.4f}; profile loss gpu={Lg:.1f} (Q*={qg:.3f}, "
-          f"final Q={Qg:.3f}) oracle={Lo:.1f} (Q*={qo:.3f}, final Q={Qo:.3f})", flush=True)
-    assert abs(pg - po) <= 0.01
-    assert abs(Lg - Lo) <= 0.02 * abs(Lo)
+    Lg = oracle.nce_loss(Y, Qg, src, dst, neg, vis.a, vis.b, grad=False)
+    Pg, _ = profile_loss(Y, src, dst, neg, vis.a, vis.b)
+    po, Lo, Po = [], [], []
+    for seed in (42, 43, 44):
+        Yo, _, Qo = oracle.fit_transform(X, d=2, n_threads=8, n_neighbors=15, random_seed=seed,
+                                         n_init_epochs=20, a=vis.a, b=vis.b, n_noise=plan)
+        po.append(knn_preservation(X, Yo, queries=q))
+        Lo.append(oracle.nce_loss(Yo, Qo, src, dst, neg, vis.a, vis.b, grad=False))
+        Po.append(profile_loss(Yo, src, dst, neg, vis.a, vis.b)[0])
+    print(f"preservation gpu={pg:.4f} oracle={np.round(po, 4)}; final-Q loss gpu={Lg:.1f} (Q={Qg:.3f}) "
+          f"oracle={np.round(Lo, 1)}; profile loss gpu={Pg:.1f} oracle={np.round(Po, 1)}", flush=True)
+    assert min(po) - 0.01 <= pg <= max(po) + 0.01
+    assert min(Lo) * 0.98 <= Lg <= max(Lo) * 1.02
+    assert min(Po) * 0.98 <= Pg <= max(Po) * 1.02

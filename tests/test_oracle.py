@@ -302,3 +302,105 @@ def test_optimize_deterministic_single_thread():
     b = oracle.optimize(Y0, 0.0, row_ptr, col, plan, 0.77, 1.22, 1.0, 1.0, 42, 1)
     assert np.array_equal(a[0], b[0]) and a[1] == b[1]
     assert np.all(np.isfinite(a[0]))
+
+
+# ---------------------------------------------------------------- T > 1 chains (alpha = 0)
+# With alpha = 0 every step is 0 (ncvis.cpp:287), Y never changes, and each thread's Q copy
+# depends only on its edge chunk, its own pcg64(seed + t) stream and the epoch's starting Q.
+# The T-thread run is then deterministic up to the float order of the Q sum, which pins the
+# chain partition (:290-291), the per-thread seeding (:281-283) and the Q average (:342-349).
+A_DEF, B_DEF = 0.7668736385558027, 1.2225638162909107
+
+
+def _small_graph(N=60, k=5, seed=11):
+    X, _ = synthetic.digits_like(N=N, D=8, seed=seed)
+    inds, dists = oracle.knn_bruteforce(X, k)
+    row_ptr, col, _ = oracle.symmetrize(inds, dists)
+    Y0 = oracle.init_embedding(N, 2, row_ptr, col, np.float32(1.0 / k), 3, 42, 1)
+    return row_ptr, col, Y0
+
+
+def _restated_q_trace(Y, row_ptr, col, plan, T, seed, Q0=0.0):
+    """Independent restatement of the T-thread epoch structure at alpha = 0: libgomp's static
+    schedule (the first E % T threads take one extra edge), thread t's negatives are the draws
+    of uniform_int(0, N-1) on pcg64(seed + t) continued across epochs (redrawn while equal to the
+    source), and Q after an epoch is the float32 mean of the T copies. The per-sample Q update is
+    the oracle's own sgd_sample (pinned by finite differences)."""
+    N = Y.shape[0]
+    src = np.repeat(np.arange(N), np.diff(row_ptr))
+    E = len(src)
+    q, r = divmod(E, T)
+    bounds = []
+    for t in range(T):
+        s0 = t * q + min(t, r)
+        bounds.append((s0, s0 + q + (1 if t < r else 0)))
+    need = E * int(max(plan)) * len(plan) + 64
+    draws = [iter(oracle.uniform_index_stream(seed + t, N, need).tolist()) for t in range(T)]
+    Qs, trace = np.float32(Q0), []
+    for n in plan:
+        copies = []
+        for t, (s0, e0) in enumerate(bounds):
+            Qc = np.float32(Qs)
+            for e in range(s0, e0):
+                i = int(src[e])
+                for j in range(int(n) + 1):
+                    if j == 0:
+                        o = int(col[e])
+                    else:
+                        o = next(draws[t])
+                        while o == i:
+                            o = next(draws[t])
+                    _, Qc = oracle.sgd_sample(Y, i, o, j == 0, int(n), float(Qc), 0.0, A_DEF, B_DEF, 1.0)
+                    Qc = np.float32(Qc)
+            copies.append(Qc)
+        Qs = np.float32(np.sum(np.array(copies, np.float32), dtype=np.float32) / np.float32(T))
+        trace.append(float(Qs))
+    return np.array(trace, np.float32)
+
+
+@pytest.mark.parametrize("T", [3, 8])
+def test_optimize_threads_alpha0_matches_restatement(T):
+    row_ptr, col, Y0 = _small_graph()
+    plan = [2, 1, 3]
+    Y, Q, trace = oracle.optimize(Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 0.0, 1.0, 42, T, trace=True)
+    assert np.array_equal(Y, Y0)
+    want = _restated_q_trace(Y0, row_ptr, col, plan, T, 42)
+    np.testing.assert_allclose(trace, want, rtol=2e-6, atol=1e-6)
+    assert Q == trace[-1]
+
+
+@pytest.mark.parametrize("T", [1, 3, 8, 50])
+def test_optimize_chains_equals_threads_at_alpha0(T):
+    row_ptr, col, Y0 = _small_graph(N=120)
+    plan = [3, 2, 5, 1]
+    _, _, t_thr = oracle.optimize(Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 0.0, 1.0, 42, T, trace=True)
+    Yc, _, t_ch = oracle.optimize_chains(Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 0.0, 1.0, 42, T)
+    assert np.array_equal(Yc, Y0)
+    np.testing.assert_allclose(t_ch, t_thr, rtol=2e-6, atol=1e-6)
+
+
+def test_optimize_threads_first_epoch_is_mean_of_chunk_runs():
+    # one epoch: Q = mean over t of a one-thread run on chunk t's edges alone, seeded seed + t
+    row_ptr, col, Y0 = _small_graph(N=90)
+    T, seed, n = 4, 42, 3
+    E = len(col)
+    _, Q, _ = oracle.optimize(Y0, 0.0, row_ptr, col, [n], A_DEF, B_DEF, 0.0, 1.0, seed, T, trace=True)
+    qs = []
+    for t in range(T):
+        s0, e0 = oracle.static_chunk(E, T, t)
+        rp = np.clip(row_ptr, s0, e0) - s0
+        _, qt = oracle.optimize(Y0, 0.0, rp, col[s0:e0], [n], A_DEF, B_DEF, 0.0, 1.0, seed + t, 1)
+        qs.append(qt)
+    assert abs(Q - np.float32(np.mean(np.array(qs, np.float64)))) < 2e-6 * max(1.0, abs(Q))
+
+
+def test_optimize_chains_single_chain_equals_one_thread():
+    # alpha != 0: one chain is the one-thread run. Both are compiled with -ffast-math (the
+    # reference's flags), which may contract the two inlined copies of the sample body
+    # differently, so they agree to float rounding rather than bit for bit.
+    row_ptr, col, Y0 = _small_graph(N=150)
+    plan = [1, 4, 2]
+    Ya, Qa = oracle.optimize(Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 1.0, 1.0, 42, 1)
+    Yb, Qb, _ = oracle.optimize_chains(Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 1.0, 1.0, 42, 1)
+    np.testing.assert_allclose(Yb, Ya, atol=2e-5, rtol=0)
+    assert abs(Qa - Qb) < 2e-5 * max(1.0, abs(Qa))
