@@ -120,6 +120,20 @@ int ncvis_optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const in
                           float alpha_Q, uint64_t seed, int64_t n_threads, int atomic_updates, float *q_trace,
                           void *stream);
 
+/* ncvis_optimize_device without any device allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments and results as ncvis_optimize_device;
+ * its scratch (chain states, Q copies, the per-epoch Q buffer, the graph-check flag) lives in one
+ * static device array per GPU (3.75 MiB, held until the process ends). Limits: at most 2^17
+ * chains (n_threads < 1 picks the GPU-sized count, which stays within it by default) and at
+ * most 2^16 epochs; larger requests return NCVIS_EINVAL before any GPU work. Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (other host threads, other streams)
+ * race and must be serialised by the caller. */
+int ncvis_optimize_device_noalloc(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col,
+                                  int64_t n_edges, float *dY, float *Q, int n_epochs, const size_t *n_noise, float a,
+                                  float b, float alpha, float alpha_Q, uint64_t seed, int64_t n_threads,
+                                  int atomic_updates, float *q_trace, void *stream);
+
 /* NCE loss over explicit negatives d_neg [E, n] int32. d_loss / d_gradQ: device double;
  * d_gradY: device float [N, d], accumulated into (may be NULL). The graph and the negatives are
  * validated first (indices in [0, N)); a bad index returns NCVIS_EINVAL. */

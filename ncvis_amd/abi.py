@@ -16,7 +16,7 @@ EXPORTS = (
     "ncvis_create", "ncvis_destroy", "ncvis_last_error", "ncvis_set_option", "ncvis_fit_transform",
     "ncvis_fit_transform_device", "ncvis_get_stats", "ncvis_get_last_error", "ncvis_knn_device",
     "ncvis_symmetrize_device", "ncvis_init_embedding_device", "ncvis_optimize_device",
-    "ncvis_nce_loss_device", "ncvis_tune", "ncvis_fit_transform_many",
+    "ncvis_nce_loss_device", "ncvis_tune", "ncvis_fit_transform_many", "ncvis_optimize_device_noalloc",
 )
 
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
@@ -59,6 +59,7 @@ def lib():
                                             C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_size_t),
                                             C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint64,
                                             C.c_int64, C.c_int, vp, vp]
+        L.ncvis_optimize_device_noalloc.argtypes = L.ncvis_optimize_device.argtypes
         L.ncvis_tune.argtypes = [C.c_char_p, C.c_double]
         L.ncvis_nce_loss_device.argtypes = [C.c_size_t, C.c_size_t, vp, vp, C.c_int64, vp, C.c_int,
                                             vp, C.c_float, C.c_float, C.c_float, vp, vp, vp, vp]
@@ -106,15 +107,16 @@ def init_embedding(N: int, d: int, d_row_ptr: int, d_col: int, alpha: float, n_i
 
 def optimize(N: int, d: int, d_row_ptr: int, d_col: int, E: int, dY: int, Q: float, n_noise,
              a: float, b: float, alpha: float, alpha_Q: float, seed: int, n_threads: int,
-             atomic_updates: int, stream=None):
-    """Runs len(n_noise) epochs; returns (final Q, per-epoch Q list)."""
+             atomic_updates: int, stream=None, noalloc: bool = False):
+    """Runs len(n_noise) epochs; returns (final Q, per-epoch Q list). noalloc: the entry point
+    that allocates no device memory (ncvis_optimize_device_noalloc)."""
     n_epochs = len(n_noise)
     arr = (C.c_size_t * max(n_epochs, 1))(*[int(v) for v in n_noise])
     trace = (C.c_float * max(n_epochs, 1))()
     q = C.c_float(Q)
-    _check(lib().ncvis_optimize_device(N, d, d_row_ptr, d_col, E, dY, C.byref(q), n_epochs, arr, a,
-                                       b, alpha, alpha_Q, seed, n_threads, atomic_updates,
-                                       C.cast(trace, vp), _stream(stream)))
+    fn = lib().ncvis_optimize_device_noalloc if noalloc else lib().ncvis_optimize_device
+    _check(fn(N, d, d_row_ptr, d_col, E, dY, C.byref(q), n_epochs, arr, a, b, alpha, alpha_Q, seed,
+              n_threads, atomic_updates, C.cast(trace, vp), _stream(stream)))
     return q.value, [trace[i] for i in range(n_epochs)]
 
 

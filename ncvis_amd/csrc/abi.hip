@@ -160,6 +160,77 @@ static void fit_device(const ncvis_handle *h, const float *dX, size_t N, size_t 
     out.chains = T;
 }
 
+// ncvis_optimize_device with its device scratch supplied (ws, err) or allocated (nullptr).
+static void optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t n_edges,
+                            float *dY, float *Q, int n_epochs, const size_t *n_noise, float a, float b, float alpha,
+                            float alpha_Q, uint64_t seed, int64_t n_threads, int atomic_updates, float *q_trace,
+                            hipStream_t st, const SgdWorkspace *ws, int *d_err) {
+    static_assert(sizeof(size_t) == sizeof(uint64_t), "n_noise is read as uint64_t");
+    if (!d_row_ptr || !d_col || !dY || !Q) throw std::invalid_argument("[ncvis_amd] optimize: null pointer");
+    if (n_epochs > 0 && !n_noise) throw std::invalid_argument("[ncvis_amd] optimize: null n_noise");
+    for (int i = 0; i < n_epochs; ++i)
+        if ((uint64_t)n_noise[i] >= 0xFFFFFFFFull) throw std::invalid_argument("[ncvis_amd] n_noise too large");
+    if (N < 2) throw std::invalid_argument("[ncvis_amd] optimize: N must be >= 2");
+    check_graph(d_row_ptr, (int64_t)N, d_col, n_edges, nullptr, 0, st, d_err);
+    SgdParams p;
+    p.a = a;
+    p.b = b;
+    p.alpha = alpha;
+    p.alpha_Q = alpha_Q;
+    p.seed = seed;
+    p.n_epochs = std::max(n_epochs, 0);
+    p.n_noise = reinterpret_cast<const uint64_t *>(n_noise);
+    p.chains = n_threads < 1 ? auto_chains(n_edges) : n_threads;
+    p.atomic_updates = atomic_updates;
+    optimize_run((int64_t)N, (int)d, d_row_ptr, d_col, n_edges, dY, Q, p, st, q_trace, nullptr, ws);
+}
+
+// ---- scratch of ncvis_optimize_device_noalloc ---------------------------------------------
+// One file-scope device array holds the epoch loop's scratch for callers that cannot allocate:
+// pcg64 states, chain start rows and Q copies for up to kNoAllocMaxChains chains, the per-epoch
+// Q buffer for up to kNoAllocMaxEpochs epochs and the graph-check flag (3.75 MiB per device,
+// held until the process ends). Every call on a device shares that device's copy.
+constexpr int64_t kNoAllocMaxChains = 1 << 17;
+constexpr int kNoAllocMaxEpochs = 1 << 16;
+constexpr size_t kAlign = 256;
+constexpr size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+constexpr size_t kOffRng = 0;
+constexpr size_t kOffChainRow = align_up(kOffRng + sizeof(U128) * kNoAllocMaxChains);
+constexpr size_t kOffQpart = align_up(kOffChainRow + sizeof(int64_t) * kNoAllocMaxChains);
+constexpr size_t kOffQdev = align_up(kOffQpart + sizeof(float) * kNoAllocMaxChains);
+constexpr size_t kOffErr = align_up(kOffQdev + sizeof(float) * (1 + (size_t)kNoAllocMaxEpochs));
+constexpr size_t kScratchBytes = align_up(kOffErr + sizeof(int));
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on the device that owns `st` (the current device for the null
+// stream), looked up once per device with that device current. The array's name in host code is
+// a host-side stand-in, never a device address: only this pointer is passed to kernels and copies.
+static unsigned char *noalloc_scratch(hipStream_t st) {
+    static std::mutex mu;
+    static std::vector<unsigned char *> ptrs;
+    int cur = 0;
+    NCVIS_HIP_CHECK(hipGetDevice(&cur));
+    int dev = cur;
+    if (st) {
+        hipDevice_t sd = 0;
+        NCVIS_HIP_CHECK(hipStreamGetDevice(st, &sd));
+        dev = (int)sd;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (dev >= (int)ptrs.size()) ptrs.resize((size_t)dev + 1, nullptr);
+    if (!ptrs[dev]) {
+        if (dev != cur) NCVIS_HIP_CHECK(hipSetDevice(dev));
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (dev != cur) NCVIS_HIP_CHECK(hipSetDevice(cur));
+        if (e != hipSuccess)
+            throw HipError(std::string("[ncvis_amd] cannot locate the no-allocation scratch: ") + hipGetErrorString(e));
+        ptrs[dev] = (unsigned char *)p;
+    }
+    return ptrs[dev];
+}
+
 extern "C" {
 
 int ncvis_create(size_t d, size_t n_threads, size_t n_neighbors, size_t M, size_t ef_construction, size_t random_seed,
@@ -406,26 +477,28 @@ int ncvis_optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const in
                           float alpha_Q, uint64_t seed, int64_t n_threads, int atomic_updates, float *q_trace,
                           void *stream) {
     return guarded(g_last_error, [&] {
-        if (!d_row_ptr || !d_col || !dY || !Q) throw std::invalid_argument("[ncvis_amd] optimize: null pointer");
-        if (n_epochs > 0 && !n_noise) throw std::invalid_argument("[ncvis_amd] optimize: null n_noise");
-        std::vector<uint64_t> nn((size_t)std::max(n_epochs, 0));
-        for (int i = 0; i < n_epochs; ++i) {
-            if ((uint64_t)n_noise[i] >= 0xFFFFFFFFull) throw std::invalid_argument("[ncvis_amd] n_noise too large");
-            nn[i] = (uint64_t)n_noise[i];
-        }
-        if (N < 2) throw std::invalid_argument("[ncvis_amd] optimize: N must be >= 2");
-        check_graph(d_row_ptr, (int64_t)N, d_col, n_edges, nullptr, 0, (hipStream_t)stream);
-        SgdParams p;
-        p.a = a;
-        p.b = b;
-        p.alpha = alpha;
-        p.alpha_Q = alpha_Q;
-        p.seed = seed;
-        p.n_epochs = n_epochs;
-        p.n_noise = nn.data();
-        p.chains = n_threads < 1 ? auto_chains(n_edges) : n_threads;
-        p.atomic_updates = atomic_updates;
-        optimize_run((int64_t)N, (int)d, d_row_ptr, d_col, n_edges, dY, Q, p, (hipStream_t)stream, q_trace);
+        optimize_device(N, d, d_row_ptr, d_col, n_edges, dY, Q, n_epochs, n_noise, a, b, alpha, alpha_Q, seed,
+                        n_threads, atomic_updates, q_trace, (hipStream_t)stream, nullptr, nullptr);
+    });
+}
+
+int ncvis_optimize_device_noalloc(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col,
+                                  int64_t n_edges, float *dY, float *Q, int n_epochs, const size_t *n_noise, float a,
+                                  float b, float alpha, float alpha_Q, uint64_t seed, int64_t n_threads,
+                                  int atomic_updates, float *q_trace, void *stream) {
+    return guarded(g_last_error, [&] {
+        const int64_t T = n_threads < 1 ? auto_chains(n_edges) : n_threads;
+        if (T > kNoAllocMaxChains)
+            throw std::invalid_argument("[ncvis_amd] optimize_noalloc: " + std::to_string(T) +
+                                        " chains exceed the scratch limit of " + std::to_string(kNoAllocMaxChains));
+        if (n_epochs > kNoAllocMaxEpochs)
+            throw std::invalid_argument("[ncvis_amd] optimize_noalloc: " + std::to_string(n_epochs) +
+                                        " epochs exceed the scratch limit of " + std::to_string(kNoAllocMaxEpochs));
+        unsigned char *base = noalloc_scratch((hipStream_t)stream);
+        SgdWorkspace ws{(int64_t *)(base + kOffChainRow), (U128 *)(base + kOffRng), (float *)(base + kOffQpart),
+                        (float *)(base + kOffQdev)};
+        optimize_device(N, d, d_row_ptr, d_col, n_edges, dY, Q, n_epochs, n_noise, a, b, alpha, alpha_Q, seed, T,
+                        atomic_updates, q_trace, (hipStream_t)stream, &ws, (int *)(base + kOffErr));
     });
 }
 

@@ -531,16 +531,26 @@ static void launch_epoch(int d, dim3 grid, float *Y, const int64_t *row_ptr, con
 }
 
 void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t E, float *dY, float *Q,
-                  const SgdParams &p, hipStream_t st, float *q_trace, double *epoch_kernel_ms) {
+                  const SgdParams &p, hipStream_t st, float *q_trace, double *epoch_kernel_ms,
+                  const SgdWorkspace *ws) {
     if (N < 2 || d < 1) throw HipError("[ncvis_amd] optimize: invalid sizes");
     const int64_t T = p.chains < 1 ? 1 : p.chains;
-    DevBuf<int64_t> chain_row((size_t)T);
-    DevBuf<U128> rng((size_t)T);
-    DevBuf<float> qpart((size_t)T);
-    DevBuf<float> qdev(1 + (size_t)std::max(p.n_epochs, 1));
-    k_chain_rows<<<cdiv(T, 256), 256, 0, st>>>(d_row_ptr, N, E, T, chain_row.p, rng.p, p.seed);
+    DevBuf<int64_t> own_chain_row;
+    DevBuf<U128> own_rng;
+    DevBuf<float> own_qpart, own_qdev;
+    SgdWorkspace w;
+    if (ws) {
+        w = *ws;
+    } else {
+        own_chain_row.alloc((size_t)T);
+        own_rng.alloc((size_t)T);
+        own_qpart.alloc((size_t)T);
+        own_qdev.alloc(1 + (size_t)std::max(p.n_epochs, 1));
+        w = SgdWorkspace{own_chain_row.p, own_rng.p, own_qpart.p, own_qdev.p};
+    }
+    k_chain_rows<<<cdiv(T, 256), 256, 0, st>>>(d_row_ptr, N, E, T, w.chain_row, w.rng, p.seed);
     NCVIS_LAUNCH_CHECK();
-    NCVIS_HIP_CHECK(hipMemcpyAsync(qdev.p, Q, sizeof(float), hipMemcpyHostToDevice, st));
+    NCVIS_HIP_CHECK(hipMemcpyAsync(w.qdev, Q, sizeof(float), hipMemcpyHostToDevice, st));
     dim3 grid((unsigned)cdiv(T, 256));
     // Optional per-epoch kernel timing: one event pair around each epoch kernel on `st`.
     std::vector<hipEvent_t> ev;
@@ -568,17 +578,17 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
         c.noise_f = (float)p.n_noise[epoch];
         c.fast_math = 0;
         if (epoch_kernel_ms) NCVIS_HIP_CHECK(hipEventRecord(ev[2 * epoch], st));
-        launch_epoch(d, grid, dY, d_row_ptr, d_col, E, T, chain_row.p, rng.p, qdev.p, qpart.p, c, (uint64_t)N,
+        launch_epoch(d, grid, dY, d_row_ptr, d_col, E, T, w.chain_row, w.rng, w.qdev, w.qpart, c, (uint64_t)N,
                      p.atomic_updates, st);
         NCVIS_LAUNCH_CHECK();
         if (epoch_kernel_ms) NCVIS_HIP_CHECK(hipEventRecord(ev[2 * epoch + 1], st));
-        k_q_reduce<<<1, 1024, 0, st>>>(qpart.p, T, qdev.p, qdev.p + 1, epoch);
+        k_q_reduce<<<1, 1024, 0, st>>>(w.qpart, T, w.qdev, w.qdev + 1, epoch);
         NCVIS_LAUNCH_CHECK();
     }
-    NCVIS_HIP_CHECK(hipMemcpyAsync(Q, qdev.p, sizeof(float), hipMemcpyDeviceToHost, st));
+    NCVIS_HIP_CHECK(hipMemcpyAsync(Q, w.qdev, sizeof(float), hipMemcpyDeviceToHost, st));
     if (q_trace && p.n_epochs > 0)
         NCVIS_HIP_CHECK(
-            hipMemcpyAsync(q_trace, qdev.p + 1, sizeof(float) * p.n_epochs, hipMemcpyDeviceToHost, st));
+            hipMemcpyAsync(q_trace, w.qdev + 1, sizeof(float) * p.n_epochs, hipMemcpyDeviceToHost, st));
     NCVIS_HIP_CHECK(hipStreamSynchronize(st));
     if (epoch_kernel_ms) {
         double tot = 0.0;
@@ -608,16 +618,20 @@ __global__ void k_check_graph(const int64_t *__restrict__ row_ptr, int64_t N, co
 }
 
 void check_graph(const int64_t *d_row_ptr, int64_t N, const int32_t *d_col, int64_t E, const int32_t *d_neg,
-                 int64_t n_neg, hipStream_t st) {
+                 int64_t n_neg, hipStream_t st, int *d_err) {
     if (E > 0 && !d_col) throw std::invalid_argument("[ncvis_amd] edges given without a column array");
-    DevBuf<int> err(1);
-    NCVIS_HIP_CHECK(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    DevBuf<int> own;
+    if (!d_err) {
+        own.alloc(1);
+        d_err = own.p;
+    }
+    NCVIS_HIP_CHECK(hipMemsetAsync(d_err, 0, sizeof(int), st));
     const int64_t work = std::max<int64_t>(std::max<int64_t>(N, E), n_neg);
     const int nb = (int)std::min<int64_t>(4096, std::max<int64_t>(1, cdiv(work, 256)));
-    k_check_graph<<<nb, 256, 0, st>>>(d_row_ptr, N, d_col, E, d_neg, n_neg, err.p);
+    k_check_graph<<<nb, 256, 0, st>>>(d_row_ptr, N, d_col, E, d_neg, n_neg, d_err);
     NCVIS_LAUNCH_CHECK();
     int herr = 0;
-    NCVIS_HIP_CHECK(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    NCVIS_HIP_CHECK(hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
     NCVIS_HIP_CHECK(hipStreamSynchronize(st));
     if (herr & 1) throw std::invalid_argument("[ncvis_amd] row_ptr must start at 0 and end at the edge count");
     if (herr & 2) throw std::invalid_argument("[ncvis_amd] row_ptr must be non-decreasing");

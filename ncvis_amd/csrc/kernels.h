@@ -67,17 +67,28 @@ struct SgdParams {
 void init_embedding_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, float alpha,
                         int n_init_epochs, uint64_t seed, int64_t chains, float *dY, hipStream_t st);
 
+// Device scratch of the epoch loop, for callers that bring their own memory: chain_row [T],
+// rng [T] (16-byte aligned), qpart [T], qdev [1 + n_epochs].
+struct SgdWorkspace {
+    int64_t *chain_row;
+    U128 *rng;
+    float *qpart;
+    float *qdev;
+};
+
 // Runs all epochs; Q (host in/out). `q_trace` (optional, host [n_epochs]) receives Q per epoch;
-// `epoch_kernel_ms` (optional, host) the summed event time of the epoch kernels alone.
+// `epoch_kernel_ms` (optional, host) the summed event time of the epoch kernels alone. `ws`
+// (optional) supplies the device scratch; without it the scratch is allocated here.
 void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t E, float *dY,
                   float *Q, const SgdParams &p, hipStream_t st, float *q_trace = nullptr,
-                  double *epoch_kernel_ms = nullptr);
+                  double *epoch_kernel_ms = nullptr, const SgdWorkspace *ws = nullptr);
 
 // Validates a caller-supplied graph before a kernel indexes with it: row_ptr [N+1] starts at 0,
 // ends at E and never decreases; col [E] and neg [n_neg] (may be null when n_neg = 0) lie in
-// [0, N). Throws std::invalid_argument otherwise. Synchronises `st`.
+// [0, N). Throws std::invalid_argument otherwise. Synchronises `st`. `d_err` (optional) is one
+// device int of caller memory for the result flags; without it one is allocated here.
 void check_graph(const int64_t *d_row_ptr, int64_t N, const int32_t *d_col, int64_t E, const int32_t *d_neg,
-                 int64_t n_neg, hipStream_t st);
+                 int64_t n_neg, hipStream_t st, int *d_err = nullptr);
 
 // NCE loss and adjoint over an explicit negative table neg[E, n] (int32). dY (float, [N,d],
 // accumulated) and dQ / loss (double) are device pointers; dY may be null for loss only.
