@@ -47,6 +47,10 @@ void knn_tiles_q1_fp16(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, 
                        int32_t *cand, hipStream_t st);
 void knn_tiles_wide(int prec, int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn,
                     float beta, int32_t *cand, hipStream_t st);  // 16-bit, DK 10/12/16 (knn_wide.hip)
+// FP32 rows of 257..512 elements (DK 10/12/16, KL 16/32): 16x16x4 tiles, four lists of KL per
+// query, cand [N, 4*KL], Npad a multiple of 64 (knn_wide_f32.hip)
+void knn_tiles_wide_f32(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
+                        int32_t *cand, hipStream_t st);
 
 // ---- symmetrize (symmetrize.hip) ---------------------------------------------------------
 // KNNTable::symmetrize at one thread, bit-exact, then flattened to CSR (= build_edges order).

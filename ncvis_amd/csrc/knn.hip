@@ -17,7 +17,9 @@
 //                   candidate rows stream through double-buffered LDS stages; every 32x32 MFMA
 //                   block is scored (cn + beta * <q, c>) and filtered into per-lane sorted top-KL
 //                   lists (two half-lists per query, one per lane half)
-//   k_knn_rerank  - exact FP32 distances for the 2*KL candidates of each query, rank by
+//                   (FP32 rows of 257..512 elements: 16x16x4 tiles with four quarter-lists per
+//                   query, knn_wide_f32.hip)
+//   k_knn_rerank  - exact FP32 distances for the 2*KL (4*KL) candidates of each query, rank by
 //                   (distance, label), drop rank 0, write ranks k..1 (farthest first)
 //   k_knn_generic - the remaining shapes: exact FP32 scan (LDS-tiled) with a max-heap per query
 #include <algorithm>
@@ -166,10 +168,11 @@ __global__ __launch_bounds__(256) void k_pack_params(const double *__restrict__ 
 
 // Element k of a padded row as stored for MFMA: (x_k - c_k) * s, rounded to the GEMM precision.
 // 16-bit precisions keep the natural order (lane half h of a 32x32x16 step reads
-// k = 16 s + 8 h .. +7, contiguous). FP32 (32x32x2: lane half h reads k = 2 s + h) is stored
-// permuted, position h*(Dpad/2) + s, so a lane's operands are contiguous too. A and B use the
-// same permutation, so dot products are unchanged.
-template <int P>
+// k = 16 s + 8 h .. +7, contiguous). FP32 is stored permuted so a lane's operands are contiguous
+// too: with G lane groups splitting each step (32x32x2: G = 2, group h reads k = 2 s + h;
+// 16x16x4: G = 4, group g reads k = 4 s + g) element k = G s + g sits at position g (Dpad / G) + s.
+// A and B use the same permutation, so dot products are unchanged.
+template <int P, int G = 2>
 __global__ void k_pack(const float *__restrict__ Xp, int64_t N, int64_t Npad, int D, int Dpad, int is_l2,
                        const float *__restrict__ cs, void *__restrict__ Xh, float *__restrict__ cn) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,10 +181,10 @@ __global__ void k_pack(const float *__restrict__ Xp, int64_t N, int64_t Npad, in
     float s = 0.f;
     if (P == PREC_FP32) {
         float *o = (float *)Xh + i * Dpad;
-        const int half = Dpad / 2;
+        const int part = Dpad / G;
         for (int p = 0; p < Dpad; ++p) {
-            int h = p / half, st = p - h * half;
-            int kk = 2 * st + h;
+            int h = p / part, st = p - h * part;
+            int kk = G * st + h;
             float v = (i < N && kk < D) ? __fmul_rn(__fsub_rn(Xp[i * D + kk], cs[kk]), sc) : 0.f;
             o[p] = v;
             s = __fadd_rn(s, __fmul_rn(v, v));
@@ -512,16 +515,19 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
     NCVIS_HIP_CHECK(hipEventCreate(&e0));
     NCVIS_HIP_CHECK(hipEventCreate(&e1));
     NCVIS_HIP_CHECK(hipEventCreate(&e2));
-    // 16-bit GEMMs tile rows up to 512 elements, FP32 up to 256; wider rows or k + 1 > 32 take
-    // the generic exact scan
-    const bool wide = (prec != PREC_FP32) && (D > 256) && (D <= 512);
+    // rows up to 256 elements: 32x32 MFMA tiles in every precision (k + 1 <= 64); rows of
+    // 257..512: one query set per wave in the 16-bit GEMMs, 16x16x4 tiles in FP32 (k + 1 <= 32);
+    // wider rows or longer lists take the generic exact scan
+    const bool wide = (D > 256) && (D <= 512);
+    const bool wide32 = wide && prec == PREC_FP32;
     const int KL = (k + 1 <= 16) ? 16 : (k + 1 <= 32) ? 32 : 64;
     const bool tiled = (D <= 256 && k + 1 <= 64) || (wide && k + 1 <= 32);
     if (tiled) {
         const int DK = D <= 32 ? 1 : D <= 64 ? 2 : D <= 128 ? 4 : D <= 256 ? 8 : D <= 320 ? 10 : D <= 384 ? 12 : 16;
         const int Dpad = 32 * DK;
         const int QB = (wide || KL >= 32 || prec == PREC_FP32) ? 1 : 2;
-        const int64_t qtile = 4 * 32 * QB;
+        const int64_t qtile = wide32 ? 64 : 4 * 32 * QB;
+        const int lists = wide32 ? 4 : 2;  // candidate lists per query
         const int64_t Npad = ((N + qtile - 1) / qtile) * qtile;
         const int esz = (prec == PREC_FP32) ? 4 : 2;
         DevBuf<unsigned char> xh((size_t)Npad * Dpad * esz);
@@ -530,7 +536,10 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
         gemm_input_transform(Xp, N, D, is_l2, cs.p, st);
         switch (prec) {
             case PREC_FP32:
-                k_pack<PREC_FP32><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs.p, xh.p, cn.p);
+                if (wide32)
+                    k_pack<PREC_FP32, 4><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs.p, xh.p, cn.p);
+                else
+                    k_pack<PREC_FP32, 2><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs.p, xh.p, cn.p);
                 break;
             case PREC_BF16:
                 k_pack<PREC_BF16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs.p, xh.p, cn.p);
@@ -541,10 +550,13 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
             default: throw HipError("[ncvis_amd] knn: unknown precision");
         }
         NCVIS_LAUNCH_CHECK();
-        DevBuf<int32_t> cand((size_t)N * 2 * KL);
+        const int C = lists * KL;  // candidates per query for the rerank: 32, 64 or 128
+        DevBuf<int32_t> cand((size_t)N * C);
         const float beta = is_l2 ? -2.0f : -1.0f;
         NCVIS_HIP_CHECK(hipEventRecord(e0, st));
-        if (wide)
+        if (wide32)
+            knn_tiles_wide_f32(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
+        else if (wide)
             knn_tiles_wide(prec, KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
         else if (KL == 16)
             knn_tiles16(prec, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
@@ -559,9 +571,9 @@ void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32
         NCVIS_LAUNCH_CHECK();
         NCVIS_HIP_CHECK(hipEventRecord(e1, st));
         const int vec4 = (D % 4 == 0) && ((uintptr_t)Xp % 16 == 0);
-        if (KL == 16)
+        if (C == 32)
             k_knn_rerank<32><<<cdiv(N, 8), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand.p, k, d_inds, d_dists, err.p);
-        else if (KL == 32)
+        else if (C == 64)
             k_knn_rerank<64><<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand.p, k, d_inds, d_dists, err.p);
         else
             k_knn_rerank128<<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand.p, k, d_inds, d_dists, err.p);

@@ -150,8 +150,9 @@ def test_knn_exact_with_duplicate_rows(torch_cuda, precision):
     assert np.array_equal(gd, od)
 
 
-@pytest.mark.parametrize("shape", [(300, 300, 15), (200, 20, 40), (5, 3, 4)])
+@pytest.mark.parametrize("shape", [(300, 600, 15), (300, 300, 45), (200, 20, 70), (5, 3, 4)])
 def test_knn_generic_path(torch_cuda, shape):
+    # rows over 512 elements, FP32 rows over 256 with k + 1 > 32, k + 1 > 64: the exact scan
     N, D, k = shape
     _say(f"knn generic N={N} D={D} k={k}")
     X, _ = synthetic.digits_like(N=N, D=D, seed=7)
@@ -173,10 +174,10 @@ def test_knn_inner_product_family(torch_cuda, distance, precision):
     np.testing.assert_allclose(np.sort(gd, 1), np.sort(od, 1), atol=2e-5, rtol=1e-5)
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
 def test_knn_recall_wide_rows_cosine(torch_cuda, precision):
     # rows of 300 elements (the 100k x 300 cosine configuration's shape) take the one-set-per-wave
-    # tiles of the 16-bit GEMM
+    # tiles of the 16-bit GEMM, or the 16x16x4 tiles of the FP32 GEMM
     _say(f"knn wide cosine {precision}")
     X, _ = synthetic.normalized_mixture(N=3000, D=300, n_clusters=12, seed=3)
     gi, gd = gpu_knn(torch_cuda, X, 15, "cosine", precision)
