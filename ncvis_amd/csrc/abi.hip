@@ -3,6 +3,7 @@
 // include/ncvis_amd.h.
 #include <algorithm>
 #include <atomic>
+#include <memory>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -25,6 +26,9 @@ struct ncvis_handle {
     int atomic_updates = -1;  // -1: atomics unless a single chain
     int64_t chains_override = 0;
     int exact_threads = 0;  // 1: exactly n_threads chains (chain t = the reference's thread t)
+    int use_graph = 1;      // fit_transform_many: capture the batch in a hipGraph and replay it
+    uint64_t gen = 1;       // bumped by every option change
+    std::unique_ptr<struct FitBatch> single, batch;  // plans kept for the next fit of the same shapes
     std::string last_error;
     hipStream_t stream = nullptr;
     // stats of the last fit
@@ -64,25 +68,264 @@ static int guarded(std::string &err, F &&f) {
     }
 }
 
+static ChainRule auto_rule() {
+    const Tuning &t = tuning();
+    return ChainRule{0, t.edges_per_chain, t.min_chains, t.max_chains, -1};
+}
+
 // GPU-sized chain count: E / edges_per_chain, within [min(E / 16, min_chains), max_chains].
 // min_chains is off by default: more, shorter chains make a small graph's epochs faster (their
 // time is the per-sample latency of a long chain) but move its result away from the reference's
 // few-thread Hogwild (the reference's own 1-D two-cluster test fails at E / 16 chains).
-static int64_t auto_chains(int64_t E) {
-    const Tuning &t = tuning();
-    int64_t c = E / std::max<int64_t>(1, t.edges_per_chain);
-    c = std::max<int64_t>(c, std::min<int64_t>(E / 16, t.min_chains));
-    return std::max<int64_t>(1, std::min<int64_t>(c, t.max_chains));
-}
+static int64_t auto_chains(int64_t E) { return chain_rule(E, auto_rule()); }
 
 // n_threads is the reference's OpenMP thread count. A GPU chain is one such thread, and a few
 // sequential chains would leave the GPU idle (one chain at 1M points is ~8e9 dependent samples),
 // so the chain count is GPU-sized unless the caller asks for exactly n_threads chains.
-static int64_t resolve_chains(const ncvis_handle *h, int64_t E) {
-    if (h->chains_override > 0) return h->chains_override;
-    if (h->exact_threads && h->n_threads > 0) return (int64_t)h->n_threads;
-    return auto_chains(E);
+static ChainRule rule_of(const ncvis_handle *h) {
+    ChainRule r = auto_rule();
+    if (h->chains_override > 0)
+        r.fixed = h->chains_override;
+    else if (h->exact_threads && h->n_threads > 0)
+        r.fixed = (int64_t)h->n_threads;
+    r.atomic_opt = h->atomic_updates;
+    return r;
 }
+
+__global__ void k_zero_ints(int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+struct FitJob {
+    const float *dX;
+    size_t N, D;
+    float *dY;
+    bool operator==(const FitJob &o) const { return dX == o.dX && N == o.N && D == o.D && dY == o.dY; }
+};
+
+// ncvis.cpp:364-369 plus this implementation's limits
+static void validate_job(const FitJob &j) {
+    if (j.N == 0 || j.D == 0)
+        throw std::invalid_argument("[ncvis::NCVis::fit_transform] Dataset should have at least one element.");
+    if (j.dY == nullptr) throw std::invalid_argument("[ncvis::NCVis::fit_transform] Null pointer provided for output.");
+    if (j.dX == nullptr) throw std::invalid_argument("[ncvis_amd] null input pointer");
+    if (j.N < 2) throw std::invalid_argument("[ncvis::NCVis::fit_transform] At least two points are required.");
+    if (j.N > (size_t)INT32_MAX - 1024) throw std::invalid_argument("[ncvis_amd] more than 2^31 points");
+}
+
+// Stage events of an eagerly queued single fit.
+struct FitEvents {
+    hipEvent_t stage[5];  // start, after knn, after symmetrize, after init, end
+    KnnEvents knn;
+    std::vector<hipEvent_t> epochs;
+    FitEvents(int n_epochs) : epochs(2 * (size_t)std::max(n_epochs, 0), nullptr) {
+        for (auto &e : stage) e = nullptr;
+        knn = KnnEvents{nullptr, nullptr, nullptr};
+        for (auto &e : stage) NCVIS_HIP_CHECK(hipEventCreate(&e));
+        NCVIS_HIP_CHECK(hipEventCreate(&knn.tiles_begin));
+        NCVIS_HIP_CHECK(hipEventCreate(&knn.tiles_end));
+        NCVIS_HIP_CHECK(hipEventCreate(&knn.rerank_end));
+        for (auto &e : epochs) NCVIS_HIP_CHECK(hipEventCreate(&e));
+    }
+    ~FitEvents() {
+        for (auto e : stage) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(knn.tiles_begin);
+        (void)hipEventDestroy(knn.tiles_end);
+        (void)hipEventDestroy(knn.rerank_end);
+        for (auto e : epochs) (void)hipEventDestroy(e);
+    }
+};
+
+// A fit of one or many data sets with one parameter set: kNN and symmetrize per data set (their
+// scratch shared, in stream order), then the segmented chain counts, init and SGD epochs of all
+// data sets at once, every stage queued without host synchronisation. Its memory is one arena
+// that stays allocated for the next fit of the same shapes; a batch can be captured in a
+// hipGraph and replayed while its inputs stay the same.
+struct FitBatch {
+    std::vector<FitJob> jobs;
+    uint64_t gen = 0;  // handle and tuning generation the plan was made for
+    std::vector<int> ks;
+    std::vector<FitSeg> segs;
+    SegTotals tot;
+    DevBuf<unsigned char> arena;
+    unsigned char *knn_ws = nullptr, *sym_ws = nullptr;
+    int32_t *inds = nullptr;
+    float *dists = nullptr, *cdist = nullptr;
+    int *errs = nullptr;
+    FitSeg *d_segs = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    FitBatch() = default;
+    FitBatch(const FitBatch &) = delete;
+    FitBatch &operator=(const FitBatch &) = delete;
+    ~FitBatch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+
+    bool matches(const std::vector<FitJob> &js, uint64_t g) const { return g == gen && js == jobs; }
+
+    void plan(const ncvis_handle *h, const std::vector<FitJob> &js, uint64_t g) {
+        if (exec) {
+            (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
+        }
+        arena.release();
+        jobs = js;
+        gen = g;
+        const int B = (int)jobs.size();
+        const int d = (int)h->d;
+        const ChainRule rule = rule_of(h);
+        ks.assign(B, 0);
+        segs.assign(B, FitSeg{});
+        tot = SegTotals{};
+        auto a256 = [](size_t x) { return (x + 255) / 256 * 256; };
+        size_t knn_bytes = 0, sym_bytes = 0, nk_max = 0;
+        for (int b = 0; b < B; ++b) {
+            const FitJob &j = jobs[b];
+            size_t kk = (h->n_neighbors < j.N - 1) ? h->n_neighbors : (j.N - 1);  // ncvis.cpp:381-382
+            ks[b] = (int)(kk > 0 ? kk : 1);
+            knn_bytes = std::max(knn_bytes, knn_workspace_bytes((int64_t)j.N, (int)j.D, ks[b], h->dist, h->precision));
+            sym_bytes = std::max(sym_bytes, symmetrize_workspace_bytes((int64_t)j.N, ks[b]));
+            nk_max = std::max(nk_max, j.N * (size_t)ks[b]);
+        }
+        // layout: shared scratch, then per data set arrays, then errors and descriptors
+        size_t o = 0;
+        auto take = [&](size_t bytes) {
+            const size_t at = o;
+            o = a256(o + bytes);
+            return at;
+        };
+        const size_t o_knn = take(knn_bytes), o_sym = take(sym_bytes), o_inds = take(sizeof(int32_t) * nk_max),
+                     o_dists = take(sizeof(float) * nk_max), o_cdist = take(sizeof(float) * 2 * nk_max);
+        struct Off {
+            size_t row_ptr, col, y2, part, stats, chain_row, rng, qpart, qdev;
+        };
+        std::vector<Off> off(B);
+        for (int b = 0; b < B; ++b) {
+            const FitJob &j = jobs[b];
+            FitSeg &v = segs[b];
+            const int64_t N = (int64_t)j.N;
+            v.N = N;
+            v.nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, cdiv(N, 256)));
+            v.T_cap = chain_rule(2 * N * ks[b], rule);  // the rule never decreases with E <= 2 N k
+            v.alpha_init = (float)(1. / ks[b]);         // ncvis.cpp:418
+            v.pairs_ok = ((uintptr_t)j.dY % 8 == 0) ? 1 : 0;
+            v.row_off = tot.rows;
+            v.chain_off = tot.slots;
+            tot.rows += N;
+            tot.slots += v.T_cap;
+            tot.max_nb = std::max(tot.max_nb, v.nb);
+            off[b].row_ptr = take(sizeof(int64_t) * (N + 1));
+            off[b].col = take(sizeof(int32_t) * 2 * N * ks[b]);
+            off[b].y2 = take(h->n_init_epochs > 0 ? sizeof(float) * N * d : 0);
+            off[b].part = take(sizeof(double) * v.nb * d);
+            off[b].stats = take(sizeof(float) * 2 * d);
+            off[b].chain_row = take(sizeof(int64_t) * v.T_cap);
+            off[b].rng = take(sizeof(U128) * v.T_cap);
+            off[b].qpart = take(sizeof(float) * v.T_cap);
+            off[b].qdev = take(sizeof(float) * (1 + (size_t)std::max(h->n_epochs, 0)));
+        }
+        const size_t o_errs = take(sizeof(int) * B), o_segs = take(sizeof(FitSeg) * B);
+        arena.alloc(o);
+        unsigned char *w = arena.p;
+        knn_ws = w + o_knn;
+        sym_ws = w + o_sym;
+        inds = (int32_t *)(w + o_inds);
+        dists = (float *)(w + o_dists);
+        cdist = (float *)(w + o_cdist);
+        errs = (int *)(w + o_errs);
+        d_segs = (FitSeg *)(w + o_segs);
+        for (int b = 0; b < B; ++b) {
+            FitSeg &v = segs[b];
+            v.Y = jobs[b].dY;
+            v.Y2 = h->n_init_epochs > 0 ? (float *)(w + off[b].y2) : nullptr;
+            v.row_ptr = (const int64_t *)(w + off[b].row_ptr);
+            v.col = (const int32_t *)(w + off[b].col);
+            v.part = (double *)(w + off[b].part);
+            v.stats = (float *)(w + off[b].stats);
+            v.chain_row = (int64_t *)(w + off[b].chain_row);
+            v.rng = (U128 *)(w + off[b].rng);
+            v.qpart = (float *)(w + off[b].qpart);
+            v.qdev = (float *)(w + off[b].qdev);
+        }
+    }
+
+    void upload(hipStream_t st) {
+        for (size_t b = 0; b < segs.size(); ++b) seg_set(d_segs + b, segs[b], st);
+    }
+
+    // every stage of every data set on `st`; ev (single eager fits only) brackets the stages
+    void enqueue(const ncvis_handle *h, hipStream_t st, FitEvents *ev) {
+        const int B = (int)jobs.size();
+        const int d = (int)h->d;
+        k_zero_ints<<<cdiv(B, 256), 256, 0, st>>>(errs, B);
+        NCVIS_LAUNCH_CHECK();
+        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[0], st));
+        for (int b = 0; b < B; ++b) {
+            const FitJob &j = jobs[b];
+            knn_enqueue(j.dX, (int64_t)j.N, (int)j.D, ks[b], h->dist, h->precision, inds, dists, knn_ws, errs + b, st,
+                        ev ? &ev->knn : nullptr);
+            if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[1], st));
+            symmetrize_enqueue(inds, dists, (int64_t)j.N, ks[b], const_cast<int64_t *>(segs[b].row_ptr),
+                               const_cast<int32_t *>(segs[b].col), cdist, sym_ws, errs + b, st);
+        }
+        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[2], st));
+        seg_dims(d_segs, B, rule_of(h), d, st);
+        seg_init_embedding(d_segs, B, tot, d, h->n_init_epochs, h->random_seed, st);
+        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[3], st));
+        SgdParams p;
+        p.a = h->a;
+        p.b = h->b;
+        p.alpha = h->alpha;
+        p.alpha_Q = h->alpha_Q;
+        p.seed = h->random_seed;
+        p.n_epochs = h->n_epochs;
+        p.n_noise = h->n_noise.data();
+        p.chains = 0;  // per data set, on the device
+        p.atomic_updates = h->atomic_updates;
+        seg_optimize(d_segs, B, tot, d, 0.f, p, st, ev ? ev->epochs.data() : nullptr);  // Q starts at 0 (ncvis.cpp:417)
+        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[4], st));
+    }
+
+    void capture(const ncvis_handle *h, hipStream_t cap) {
+        upload(cap);
+        NCVIS_HIP_CHECK(hipStreamSynchronize(cap));
+        hipGraph_t g = nullptr;
+        NCVIS_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
+        try {
+            enqueue(h, cap, nullptr);
+        } catch (...) {
+            (void)hipStreamEndCapture(cap, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        NCVIS_HIP_CHECK(hipStreamEndCapture(cap, &g));
+        hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        NCVIS_HIP_CHECK(e);
+    }
+
+    // error flags of every data set (after the stream has finished); throws for the first
+    void check_errors(hipStream_t st, bool name_set) {
+        const int B = (int)jobs.size();
+        std::vector<int> herr(B, 0);
+        NCVIS_HIP_CHECK(hipMemcpyAsync(herr.data(), errs, sizeof(int) * B, hipMemcpyDeviceToHost, st));
+        NCVIS_HIP_CHECK(hipStreamSynchronize(st));
+        for (int b = 0; b < B; ++b) {
+            if (!herr[b]) continue;
+            std::string m = (herr[b] & kErrKnnCandidates)
+                                ? "[ncvis_amd] knn: fewer than k+1 finite candidates (non-finite input?)"
+                                : "[ncvis_amd] symmetrize: neighbour index out of range";
+            if (name_set) m = "[ncvis_amd] fit_transform_many: data set " + std::to_string(b) + ": " + m;
+            throw HipError(m);
+        }
+    }
+};
+
+// handle parameters and process-wide knobs change a plan: both count generations
+static std::atomic<uint64_t> g_tuning_gen{0};
+static uint64_t plan_gen(const ncvis_handle *h);
 
 struct FitStats {
     double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -91,74 +334,9 @@ struct FitStats {
     int64_t chains = 0;
 };
 
-// One whole fit (ncvis.cpp:354-459) on device buffers, queued on `st`; reads the handle's
-// parameters only, so fits of several data sets may run concurrently on their own streams.
-static void fit_device(const ncvis_handle *h, const float *dX, size_t N, size_t D, float *dY, hipStream_t st,
-                       FitStats &out) {
-    // ncvis.cpp:364-369
-    if (N == 0 || D == 0)
-        throw std::invalid_argument("[ncvis::NCVis::fit_transform] Dataset should have at least one element.");
-    if (dY == nullptr)
-        throw std::invalid_argument("[ncvis::NCVis::fit_transform] Null pointer provided for output.");
-    if (N < 2) throw std::invalid_argument("[ncvis::NCVis::fit_transform] At least two points are required.");
-    if (N > (size_t)INT32_MAX - 1024) throw std::invalid_argument("[ncvis_amd] more than 2^31 points");
-    // ncvis.cpp:381-382
-    size_t kk = (h->n_neighbors < N - 1) ? h->n_neighbors : (N - 1);
-    kk = (kk > 0) ? kk : 1;
-    const int k = (int)kk;
-    const int64_t n = (int64_t)N;
-    hipEvent_t ev[5];
-    for (auto &e : ev) NCVIS_HIP_CHECK(hipEventCreate(&e));
-    struct EvGuard {
-        hipEvent_t *e;
-        ~EvGuard() {
-            for (int i = 0; i < 5; ++i) (void)hipEventDestroy(e[i]);
-        }
-    } evg{ev};
-    NCVIS_HIP_CHECK(hipEventRecord(ev[0], st));
-    DevBuf<int32_t> inds((size_t)n * k);
-    DevBuf<float> dists((size_t)n * k);
-    KnnStats ks;
-    knn_run(dX, n, (int)D, k, h->dist, h->precision, inds.p, dists.p, st, &ks);
-    NCVIS_HIP_CHECK(hipEventRecord(ev[1], st));
-    DevBuf<int64_t> row_ptr((size_t)n + 1);
-    DevBuf<int32_t> col((size_t)2 * n * k);
-    DevBuf<float> cdist((size_t)2 * n * k);
-    const int64_t E = symmetrize_run(inds.p, dists.p, n, k, row_ptr.p, col.p, cdist.p, st);
-    inds.release();
-    dists.release();
-    cdist.release();
-    NCVIS_HIP_CHECK(hipEventRecord(ev[2], st));
-    const int64_t T = resolve_chains(h, E);
-    const float init_alpha = (float)(1. / k);  // ncvis.cpp:418
-    init_embedding_run(n, (int)h->d, row_ptr.p, col.p, init_alpha, h->n_init_epochs, h->random_seed, T, dY, st);
-    NCVIS_HIP_CHECK(hipEventRecord(ev[3], st));
-    SgdParams p;
-    p.a = h->a;
-    p.b = h->b;
-    p.alpha = h->alpha;
-    p.alpha_Q = h->alpha_Q;
-    p.seed = h->random_seed;
-    p.n_epochs = h->n_epochs;
-    p.n_noise = h->n_noise.data();
-    p.chains = T;
-    p.atomic_updates = h->atomic_updates >= 0 ? h->atomic_updates : (T > 1 ? 1 : 0);
-    float Q = 0.f;  // ncvis.cpp:417
-    double epoch_kernel_ms = 0.0;
-    optimize_run(n, (int)h->d, row_ptr.p, col.p, E, dY, &Q, p, st, nullptr, &epoch_kernel_ms);
-    NCVIS_HIP_CHECK(hipEventRecord(ev[4], st));
-    NCVIS_HIP_CHECK(hipEventSynchronize(ev[4]));
-    float ms[4];
-    for (int i = 0; i < 4; ++i) NCVIS_HIP_CHECK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
-    for (int i = 0; i < 4; ++i) out.ms[i] = ms[i];
-    out.ms[4] = ms[0] + ms[1] + ms[2] + ms[3];
-    out.ms[5] = ks.tile_ms;
-    out.ms[6] = ks.rerank_ms;
-    out.ms[7] = epoch_kernel_ms;
-    out.n_edges = E;
-    out.Q = Q;
-    out.chains = T;
-}
+// One whole fit (ncvis.cpp:354-459) on device buffers, queued on `st`, timed stage by stage.
+static void fit_device(ncvis_handle *h, const float *dX, size_t N, size_t D, float *dY, hipStream_t st,
+                       FitStats &out);
 
 // ncvis_optimize_device with its device scratch supplied (ws, err) or allocated (nullptr).
 static void optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t n_edges,
@@ -188,7 +366,8 @@ static void optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const 
 // ---- scratch of ncvis_optimize_device_noalloc ---------------------------------------------
 // One file-scope device array holds the epoch loop's scratch for callers that cannot allocate:
 // pcg64 states, chain start rows and Q copies for up to kNoAllocMaxChains chains, the per-epoch
-// Q buffer for up to kNoAllocMaxEpochs epochs and the graph-check flag (3.75 MiB per device,
+// Q buffer for up to kNoAllocMaxEpochs epochs, the graph-check flag and the data set's
+// descriptor (3.75 MiB per device,
 // held until the process ends). Every call on a device shares that device's copy.
 constexpr int64_t kNoAllocMaxChains = 1 << 17;
 constexpr int kNoAllocMaxEpochs = 1 << 16;
@@ -199,7 +378,8 @@ constexpr size_t kOffChainRow = align_up(kOffRng + sizeof(U128) * kNoAllocMaxCha
 constexpr size_t kOffQpart = align_up(kOffChainRow + sizeof(int64_t) * kNoAllocMaxChains);
 constexpr size_t kOffQdev = align_up(kOffQpart + sizeof(float) * kNoAllocMaxChains);
 constexpr size_t kOffErr = align_up(kOffQdev + sizeof(float) * (1 + (size_t)kNoAllocMaxEpochs));
-constexpr size_t kScratchBytes = align_up(kOffErr + sizeof(int));
+constexpr size_t kOffSeg = align_up(kOffErr + sizeof(int));
+constexpr size_t kScratchBytes = align_up(kOffSeg + sizeof(FitSeg));
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -229,6 +409,46 @@ static unsigned char *noalloc_scratch(hipStream_t st) {
         ptrs[dev] = (unsigned char *)p;
     }
     return ptrs[dev];
+}
+
+static uint64_t plan_gen(const ncvis_handle *h) { return (h->gen << 20) ^ g_tuning_gen.load(); }
+
+static void fit_device(ncvis_handle *h, const float *dX, size_t N, size_t D, float *dY, hipStream_t st,
+                       FitStats &out) {
+    const std::vector<FitJob> js{FitJob{dX, N, D, dY}};
+    validate_job(js[0]);
+    const uint64_t g = plan_gen(h);
+    if (!h->single) h->single.reset(new FitBatch());
+    FitBatch &fb = *h->single;
+    const bool same_shape = fb.gen == g && fb.jobs.size() == 1 && fb.jobs[0].N == N && fb.jobs[0].D == D &&
+                            (((uintptr_t)fb.jobs[0].dY % 8 == 0) == ((uintptr_t)dY % 8 == 0));
+    if (same_shape) {
+        fb.jobs = js;  // same arena, new input and output pointers
+        fb.segs[0].Y = dY;
+    } else {
+        fb.plan(h, js, g);
+    }
+    FitEvents ev(h->n_epochs);
+    fb.upload(st);
+    fb.enqueue(h, st, &ev);
+    fb.check_errors(st, false);  // synchronises st
+    FitSeg v;
+    NCVIS_HIP_CHECK(hipMemcpy(&v, fb.d_segs, sizeof(FitSeg), hipMemcpyDeviceToHost));
+    float Q = 0.f;
+    NCVIS_HIP_CHECK(hipMemcpy(&Q, fb.segs[0].qdev, sizeof(float), hipMemcpyDeviceToHost));
+    float ms[4];
+    for (int i = 0; i < 4; ++i) NCVIS_HIP_CHECK(hipEventElapsedTime(&ms[i], ev.stage[i], ev.stage[i + 1]));
+    for (int i = 0; i < 4; ++i) out.ms[i] = ms[i];
+    out.ms[4] = ms[0] + ms[1] + ms[2] + ms[3];
+    float t1 = 0.f, t2 = 0.f;
+    NCVIS_HIP_CHECK(hipEventElapsedTime(&t1, ev.knn.tiles_begin, ev.knn.tiles_end));
+    NCVIS_HIP_CHECK(hipEventElapsedTime(&t2, ev.knn.tiles_end, ev.knn.rerank_end));
+    out.ms[5] = t1;
+    out.ms[6] = t2;
+    out.ms[7] = epoch_time_ms(ev.epochs.data(), h->n_epochs);
+    out.n_edges = v.E;
+    out.Q = Q;
+    out.chains = v.T;
 }
 
 extern "C" {
@@ -288,6 +508,7 @@ int ncvis_set_option(ncvis_handle *h, const char *key, double value) {
     if (!h) return NCVIS_EINVAL;
     return guarded(h->last_error, [&] {
         std::string k = key ? key : "";
+        ++h->gen;
         if (k == "knn_precision") {
             int v = (int)value;
             if (v < PREC_FP32 || v > PREC_FP16) throw std::invalid_argument("[ncvis_amd] knn_precision must be 0, 1 or 2");
@@ -299,6 +520,8 @@ int ncvis_set_option(ncvis_handle *h, const char *key, double value) {
             h->chains_override = (int64_t)value;
         } else if (k == "exact_threads") {
             h->exact_threads = value != 0;
+        } else if (k == "graph") {
+            h->use_graph = value != 0;
         } else {
             throw std::invalid_argument("[ncvis_amd] unknown option '" + k + "'");
         }
@@ -309,6 +532,7 @@ int ncvis_tune(const char *key, double value) {
     return guarded(g_last_error, [&] {
         std::string k = key ? key : "";
         Tuning &t = tuning();
+        ++g_tuning_gen;
         if (k == "sgd_prefetch") {
             int v = (int)value;
             if (v != 1 && v != 2 && v != 4 && v != 8) throw std::invalid_argument("[ncvis_amd] sgd_prefetch: 1, 2, 4 or 8");
@@ -351,53 +575,34 @@ int ncvis_fit_transform_device(ncvis_handle *h, const float *dX, size_t N, size_
 
 int ncvis_fit_transform_many(ncvis_handle *h, size_t count, const float *const *dX, const size_t *N, const size_t *D,
                              float *const *dY, int n_streams, void *stream) {
+    (void)n_streams;  // the data sets share every segmented launch; no stream split is needed
     if (!h) return NCVIS_EINVAL;
     return guarded(h->last_error, [&] {
         if (count == 0) return;
         if (!dX || !N || !D || !dY) throw std::invalid_argument("[ncvis_amd] fit_transform_many: null array");
-        // inputs may still be in flight on the caller's stream
-        NCVIS_HIP_CHECK(hipStreamSynchronize(stream ? (hipStream_t)stream : h->stream));
-        const int S = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(n_streams, 1), count));
-        std::atomic<size_t> next{0};
-        std::mutex mu;
-        std::string first_error;
-        size_t first_index = count;
-        auto worker = [&]() {
-            hipStream_t ws = nullptr;
-            std::string err;
-            int rc = guarded(err, [&] { NCVIS_HIP_CHECK(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking)); });
-            if (rc == NCVIS_OK) {
-                // stream-ordered allocations: hipFree would wait for every stream of the device
-                alloc_stream() = ws;
-                for (size_t i = next++; i < count; i = next++) {
-                    FitStats fs;
-                    rc = guarded(err, [&] { fit_device(h, dX[i], N[i], D[i], dY[i], ws, fs); });
-                    if (rc != NCVIS_OK) {
-                        std::lock_guard<std::mutex> lk(mu);
-                        if (i < first_index) {
-                            first_index = i;
-                            first_error = err;
-                        }
-                    }
-                }
-                (void)hipStreamSynchronize(ws);
-                alloc_stream() = nullptr;
-                (void)hipStreamDestroy(ws);
-            } else {
-                std::lock_guard<std::mutex> lk(mu);
-                if (first_error.empty()) first_error = err;
+        std::vector<FitJob> js(count);
+        for (size_t i = 0; i < count; ++i) {
+            js[i] = FitJob{dX[i], N[i], D[i], dY[i]};
+            try {
+                validate_job(js[i]);
+            } catch (const std::exception &e) {
+                throw std::invalid_argument("[ncvis_amd] fit_transform_many: data set " + std::to_string(i) + ": " +
+                                            e.what());
             }
-        };
-        std::vector<std::thread> pool;
-        for (int t = 1; t < S; ++t) pool.emplace_back(worker);
-        worker();
-        for (auto &t : pool) t.join();
-        if (!first_error.empty()) {
-            if (first_index < count)
-                throw std::runtime_error("[ncvis_amd] fit_transform_many: data set " + std::to_string(first_index) +
-                                         ": " + first_error);
-            throw std::runtime_error(first_error);
         }
+        hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+        const uint64_t g = plan_gen(h);
+        if (!h->batch) h->batch.reset(new FitBatch());
+        FitBatch &fb = *h->batch;
+        if (!fb.matches(js, g)) fb.plan(h, js, g);
+        if (h->use_graph) {
+            if (!fb.exec) fb.capture(h, h->stream);
+            NCVIS_HIP_CHECK(hipGraphLaunch(fb.exec, st));
+        } else {
+            fb.upload(st);
+            fb.enqueue(h, st, nullptr);
+        }
+        fb.check_errors(st, true);
     });
 }
 
@@ -496,7 +701,7 @@ int ncvis_optimize_device_noalloc(size_t N, size_t d, const int64_t *d_row_ptr, 
                                         " epochs exceed the scratch limit of " + std::to_string(kNoAllocMaxEpochs));
         unsigned char *base = noalloc_scratch((hipStream_t)stream);
         SgdWorkspace ws{(int64_t *)(base + kOffChainRow), (U128 *)(base + kOffRng), (float *)(base + kOffQpart),
-                        (float *)(base + kOffQdev)};
+                        (float *)(base + kOffQdev), (FitSeg *)(base + kOffSeg)};
         optimize_device(N, d, d_row_ptr, d_col, n_edges, dY, Q, n_epochs, n_noise, a, b, alpha, alpha_Q, seed, T,
                         atomic_updates, q_trace, (hipStream_t)stream, &ws, (int *)(base + kOffErr));
     });

@@ -1,25 +1,31 @@
 // Layout initialisation, NCE SGD epochs and the explicit NCE loss / adjoint on gfx950.
 //
+// Every data set of a fit is one FitSeg descriptor in device memory; the init and epoch kernels
+// walk the concatenation of all data sets in one launch (a thread finds its data set by binary
+// search over the descriptors' offsets), so one fit and a batch of many share one code path and
+// a batch fills the GPU with all its data sets' rows and chains at once. Edge and chain counts
+// are read from the device (k_seg_dims), so no stage waits for the host.
+//
 // Initialisation (src/ncvis.cpp:184-273):
-//   k_init_random  Y[i] = uniform_real<float>(0,1) drawn from pcg64(seed + t), where t is the
-//                  thread that owns element i under libgomp's static schedule of the N*d loop
-//                  (ncvis.cpp:198-201); a GPU thread jumps the stream to its first element.
-//   k_spmm_rows    Y_new[i] = sum_{edges (i, j) in row order} alpha * Y_old[j], row-owned
-//                  (race-free), uncontracted like the reference's loop (ncvis.cpp:206-218)
-//   k_colsum / k_colvar / k_finalize / k_normalize
-//                  per-dimension mean and population std (FP64 partial sums, deterministic)
-//                  and standardisation (ncvis.cpp:220-260)
+//   k_init_random_seg  Y[i] = uniform_real<float>(0,1) drawn from pcg64(seed + t), where t is
+//                      the thread that owns element i under libgomp's static schedule of the N*d
+//                      loop (ncvis.cpp:198-201); a GPU thread jumps the stream to its element.
+//   k_spmm_seg         Y_new[i] = sum_{edges (i, j) in row order} alpha * Y_old[j], row-owned
+//                      (race-free), uncontracted like the reference's loop (ncvis.cpp:206-218)
+//   k_colstat_seg / k_finalize_seg / k_normalize_seg
+//                      per-dimension mean and population std (FP64 partial sums over a fixed
+//                      block partition, deterministic) and standardisation (ncvis.cpp:220-260)
 // Epochs (src/ncvis.cpp:275-352):
-//   k_sgd_epoch    T chains, chain t = the reference's OpenMP thread t: pcg64(seed + t), the
-//                  static edge chunk [s0, e0) of thread t, a private Q copy, sequential samples.
-//                  Chains run concurrently Hogwild-style; T = 1 is the reference at one thread.
-//   k_sgd_epoch_hw the many-chain form: source row kept in registers with the chain's own
-//                  updates, other endpoints updated atomically (one 64-bit compare-and-swap per
-//                  coordinate pair), negatives' rows prefetched.
-//   k_q_reduce     Q = mean of the chains' Q copies (ncvis.cpp:342-349), FP64 sum.
+//   k_sgd_epoch_seg    chain t of a data set = the reference's OpenMP thread t: pcg64(seed + t),
+//                      the static edge chunk [s0, e0) of thread t, a private Q copy. Mode 0 is
+//                      the exact form (plain loads/stores, one sample at a time; one chain is
+//                      the reference at one thread); mode 1 the same with float atomics; mode 2
+//                      the many-chain form (source row in registers with the chain's own
+//                      updates, far endpoints by 64-bit compare-and-swap per coordinate pair).
+//   k_q_reduce_seg     Q = mean of the chains' Q copies (ncvis.cpp:342-349), FP64 sum.
 // NCE loss (the objective the update of ncvis.cpp:304-338 descends):
-//   k_nce_loss     per edge: positive + n explicit negatives; loss and dL/dQ as FP64 partial
-//                  sums, dL/dY scattered with FP32 atomics.
+//   k_nce_loss         per edge: positive + n explicit negatives; loss and dL/dQ as FP64 partial
+//                      sums, dL/dY scattered with FP32 atomics.
 #include <algorithm>
 #include <cfloat>
 #include <stdexcept>
@@ -30,157 +36,248 @@
 
 namespace ncvis_hip {
 
+// ------------------------------------------------------------------------ descriptors
+__global__ void k_seg_set(FitSeg *__restrict__ dst, FitSeg v) { *dst = v; }
+
+void seg_set(FitSeg *d_seg, const FitSeg &v, hipStream_t st) {
+    k_seg_set<<<1, 1, 0, st>>>(d_seg, v);
+    NCVIS_LAUNCH_CHECK();
+}
+
+// data set holding global position x of the offsets selected by OFF (0: rows, 1: chain slots)
+template <int OFF>
+__device__ __forceinline__ int seg_find(const FitSeg *__restrict__ s, int B, int64_t x) {
+    int lo = 0, hi = B;
+    while (hi - lo > 1) {
+        const int m = (lo + hi) >> 1;
+        const int64_t o = OFF == 0 ? s[m].row_off : s[m].chain_off;
+        if (o <= x)
+            lo = m;
+        else
+            hi = m;
+    }
+    return lo;
+}
+
+// E = row_ptr[N], T by the chain rule, and the epoch mode of every data set
+__global__ void k_seg_dims(FitSeg *__restrict__ s, int B, ChainRule rule, int d) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    FitSeg &g = s[b];
+    const int64_t E = g.row_ptr[g.N];
+    int64_t T = chain_rule(E, rule);
+    if (T > g.T_cap) T = g.T_cap;
+    const int atomic = rule.atomic_opt >= 0 ? rule.atomic_opt : (T > 1 ? 1 : 0);
+    g.E = E;
+    g.T = T;
+    g.mode = atomic ? ((d >= 1 && d <= 4 && (d % 2 != 0 || g.pairs_ok)) ? 2 : 1) : 0;
+}
+
+void seg_dims(FitSeg *d_segs, int B, const ChainRule &rule, int d, hipStream_t st) {
+    k_seg_dims<<<cdiv(B, 64), 64, 0, st>>>(d_segs, B, rule, d);
+    NCVIS_LAUNCH_CHECK();
+}
+
 // ------------------------------------------------------------------------ init
-__global__ void k_init_random(float *__restrict__ Y, int64_t Nd, int64_t T, uint64_t seed) {
+// Elements [i0, i0 + 64) of the concatenated N*d element space; a run may span data sets.
+__global__ void k_init_random_seg(const FitSeg *__restrict__ s, int B, int64_t total, int d, uint64_t seed) {
     constexpr int RUN = 64;
-    int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * RUN;
-    if (i0 >= Nd) return;
-    int64_t i1 = i0 + RUN < Nd ? i0 + RUN : Nd;
-    // owner of element i0 under the static schedule
-    int64_t q = Nd / T, r = Nd % T, t;
-    if (i0 < r * (q + 1))
-        t = i0 / (q + 1);
-    else
-        t = r + (i0 - r * (q + 1)) / q;
-    int64_t s0, e0;
-    static_chunk(Nd, T, t, s0, e0);
-    U128 g = pcg_advance(pcg_seed(seed + (uint64_t)t), (uint64_t)(i0 - s0));
-    for (int64_t i = i0; i < i1; ++i) {
-        while (i >= e0) {
-            ++t;
-            static_chunk(Nd, T, t, s0, e0);
-            g = pcg_seed(seed + (uint64_t)t);
+    const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * RUN;
+    if (i0 >= total) return;
+    const int64_t i1 = i0 + RUN < total ? i0 + RUN : total;
+    int64_t i = i0;
+    while (i < i1) {
+        const int b = seg_find<0>(s, B, i / d);
+        const FitSeg &g = s[b];
+        const int64_t base = g.row_off * d, Nd = g.N * d, T = g.T;
+        const int64_t l0 = i - base;
+        const int64_t l1 = (i1 - base < Nd) ? i1 - base : Nd;
+        // owner of element l0 under the static schedule of Nd over T threads
+        const int64_t q = Nd / T, r = Nd % T;
+        int64_t t = (l0 < r * (q + 1)) ? l0 / (q + 1) : r + (l0 - r * (q + 1)) / q;
+        int64_t s0, e0;
+        static_chunk(Nd, T, t, s0, e0);
+        U128 st = pcg_advance(pcg_seed(seed + (uint64_t)t), (uint64_t)(l0 - s0));
+        for (int64_t l = l0; l < l1; ++l) {
+            while (l >= e0) {
+                ++t;
+                static_chunk(Nd, T, t, s0, e0);
+                st = pcg_seed(seed + (uint64_t)t);
+            }
+            g.Y[l] = uniform_unit_float(st);
         }
-        Y[i] = uniform_unit_float(g);
+        i = base + l1;
     }
 }
 
-__global__ void k_spmm_rows(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col, int64_t N, int d,
-                            float alpha, const float *__restrict__ Yo, float *__restrict__ Yn) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const int64_t b = row_ptr[i], e = row_ptr[i + 1];
-    for (int c = 0; c < d; ++c) {
-        float acc = 0.f;
-        for (int64_t p = b; p < e; ++p) acc = __fadd_rn(acc, __fmul_rn(alpha, Yo[(int64_t)col[p] * d + c]));
-        Yn[i * d + c] = acc;
-    }
-}
-
-// Same sums for d <= 4 with the dimension loop inside the edge loop: each edge's column index
-// and row are read once, four edges are fetched ahead of their (in-order) accumulation. Every
-// dimension's sum keeps the sequential edge order, so the result equals k_spmm_rows bit for bit.
+// One row per thread over the concatenated rows; parity 0 reads Y and writes Y2, 1 the reverse.
+// DIM 1..4: the dimension loop inside the edge loop (each edge's index and row read once, four
+// edges fetched ahead of their in-order accumulation); DIM 0: any d, one dimension at a time.
+// Every dimension's sum keeps the sequential edge order, so both forms give the same bits.
 template <int DIM>
-__global__ void k_spmm_rows_d(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col, int64_t N,
-                              float alpha, const float *__restrict__ Yo, float *__restrict__ Yn) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const int64_t b = row_ptr[i], e = row_ptr[i + 1];
-    float acc[DIM];
+__global__ void k_spmm_seg(const FitSeg *__restrict__ s, int B, int64_t R, int d_rt, int parity) {
+    const int64_t gr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gr >= R) return;
+    const FitSeg &g = s[seg_find<0>(s, B, gr)];
+    const int64_t i = gr - g.row_off;
+    const float *__restrict__ Yo = parity ? g.Y2 : g.Y;
+    float *__restrict__ Yn = parity ? g.Y : g.Y2;
+    const int32_t *__restrict__ col = g.col;
+    const float alpha = g.alpha_init;
+    const int64_t b = g.row_ptr[i], e = g.row_ptr[i + 1];
+    if constexpr (DIM == 0) {
+        for (int c = 0; c < d_rt; ++c) {
+            float acc = 0.f;
+            for (int64_t p = b; p < e; ++p) acc = __fadd_rn(acc, __fmul_rn(alpha, Yo[(int64_t)col[p] * d_rt + c]));
+            Yn[i * d_rt + c] = acc;
+        }
+    } else {
+        float acc[DIM];
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) acc[c] = 0.f;
-    int64_t p = b;
-    for (; p + 4 <= e; p += 4) {
-        int32_t j[4];
+        for (int c = 0; c < DIM; ++c) acc[c] = 0.f;
+        int64_t p = b;
+        for (; p + 4 <= e; p += 4) {
+            int32_t j[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) j[u] = col[p + u];
-        float y[4][DIM];
+            for (int u = 0; u < 4; ++u) j[u] = col[p + u];
+            float y[4][DIM];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int c = 0; c < DIM; ++c) y[u][c] = Yo[(int64_t)j[u] * DIM + c];
+                for (int c = 0; c < DIM; ++c) y[u][c] = Yo[(int64_t)j[u] * DIM + c];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int c = 0; c < DIM; ++c) acc[c] = __fadd_rn(acc[c], __fmul_rn(alpha, y[u][c]));
+                for (int c = 0; c < DIM; ++c) acc[c] = __fadd_rn(acc[c], __fmul_rn(alpha, y[u][c]));
+        }
+        for (; p < e; ++p) {
+            const int64_t j = col[p];
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) acc[c] = __fadd_rn(acc[c], __fmul_rn(alpha, Yo[j * DIM + c]));
+        }
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) Yn[i * DIM + c] = acc[c];
     }
-    for (; p < e; ++p) {
-        const int64_t j = col[p];
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) acc[c] = __fadd_rn(acc[c], __fmul_rn(alpha, Yo[j * DIM + c]));
-    }
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) Yn[i * DIM + c] = acc[c];
 }
 
-// per-block FP64 column sums of Y (mode 0) or of (Y - mean)^2 (mode 1)
+// Per-block FP64 column sums of the freshly written buffer (MODE 0) or of (Y - mean)^2 (MODE 1).
+// Grid (max nb, B): data set blockIdx.y uses its first nb blocks, each striding over its rows.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_colstat(const float *__restrict__ Y, int64_t N, int d,
-                                                 const float *__restrict__ mean, double *__restrict__ partial) {
+__global__ __launch_bounds__(256) void k_colstat_seg(const FitSeg *__restrict__ s, int d, int parity) {
     __shared__ double red[256];
+    const FitSeg &g = s[blockIdx.y];
+    if ((int)blockIdx.x >= g.nb) return;
+    const float *__restrict__ Y = parity ? g.Y : g.Y2;
+    const int64_t N = g.N;
     for (int c = 0; c < d; ++c) {
-        double s = 0.0;
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)g.nb * blockDim.x) {
             float v = Y[i * d + c];
             if (MODE == 1) {
-                float dl = __fsub_rn(v, mean[c]);
+                float dl = __fsub_rn(v, g.stats[c]);
                 v = __fmul_rn(dl, dl);
             }
-            s += (double)v;
+            acc += (double)v;
         }
-        red[threadIdx.x] = s;
+        red[threadIdx.x] = acc;
         __syncthreads();
         for (int w = 128; w > 0; w >>= 1) {
             if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * d + c] = red[0];
+        if (threadIdx.x == 0) g.part[(int64_t)blockIdx.x * d + c] = red[0];
         __syncthreads();
     }
 }
 
-// MODE 0: mean[c] = sum / N; MODE 1: sigma[c] = sqrtf(sum / N)
+// One block per data set. MODE 0: mean[c] = sum / N; MODE 1: sigma[c] = sqrtf(sum / N).
 template <int MODE>
-__global__ void k_finalize(const double *__restrict__ partial, int nb, int d, int64_t N, float *__restrict__ out) {
-    int c = threadIdx.x;
-    if (c >= d) return;
-    double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += partial[(int64_t)b * d + c];
-    float v = (float)(s / (double)N);
-    out[c] = (MODE == 0) ? v : sqrtf(v);
+__global__ void k_finalize_seg(const FitSeg *__restrict__ s, int d) {
+    const FitSeg &g = s[blockIdx.x];
+    for (int c = threadIdx.x; c < d; c += blockDim.x) {
+        double acc = 0.0;
+        for (int b = 0; b < g.nb; ++b) acc += g.part[(int64_t)b * d + c];
+        const float v = (float)(acc / (double)g.N);
+        g.stats[MODE * d + c] = (MODE == 0) ? v : sqrtf(v);
+    }
 }
 
-__global__ void k_normalize(float *__restrict__ Y, int64_t N, int d, const float *__restrict__ mean,
-                            const float *__restrict__ sigma) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N * d) return;
-    int c = (int)(i % d);
-    Y[i] = __fsub_rn(Y[i], mean[c]) / sigma[c];
+__global__ void k_normalize_seg(const FitSeg *__restrict__ s, int B, int64_t total, int d, int parity) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const FitSeg &g = s[seg_find<0>(s, B, e / d)];
+    const int64_t l = e - g.row_off * d;
+    const int c = (int)(l % d);
+    float *Y = parity ? g.Y : g.Y2;
+    Y[l] = __fsub_rn(Y[l], g.stats[c]) / g.stats[d + c];
 }
+
+__global__ void k_copy_back_seg(const FitSeg *__restrict__ s, int B, int64_t total, int d) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const FitSeg &g = s[seg_find<0>(s, B, e / d)];
+    const int64_t l = e - g.row_off * d;
+    g.Y[l] = g.Y2[l];
+}
+
+void seg_init_embedding(const FitSeg *d_segs, int B, const SegTotals &tot, int d, int n_init_epochs, uint64_t seed,
+                        hipStream_t st) {
+    const int64_t total = tot.rows * d;
+    if (total <= 0) return;
+    k_init_random_seg<<<cdiv(cdiv(total, 64), 256), 256, 0, st>>>(d_segs, B, total, d, seed);
+    NCVIS_LAUNCH_CHECK();
+    const dim3 sgrid((unsigned)std::max(1, tot.max_nb), (unsigned)B);
+    for (int ep = 0; ep < n_init_epochs; ++ep) {
+        const int par = ep % 2;  // epoch ep reads buffer par (0: Y) and writes the other
+        switch (d) {
+            case 1: k_spmm_seg<1><<<cdiv(tot.rows, 256), 256, 0, st>>>(d_segs, B, tot.rows, d, par); break;
+            case 2: k_spmm_seg<2><<<cdiv(tot.rows, 256), 256, 0, st>>>(d_segs, B, tot.rows, d, par); break;
+            case 3: k_spmm_seg<3><<<cdiv(tot.rows, 256), 256, 0, st>>>(d_segs, B, tot.rows, d, par); break;
+            case 4: k_spmm_seg<4><<<cdiv(tot.rows, 256), 256, 0, st>>>(d_segs, B, tot.rows, d, par); break;
+            default: k_spmm_seg<0><<<cdiv(tot.rows, 256), 256, 0, st>>>(d_segs, B, tot.rows, d, par); break;
+        }
+        NCVIS_LAUNCH_CHECK();
+        k_colstat_seg<0><<<sgrid, 256, 0, st>>>(d_segs, d, par);
+        k_finalize_seg<0><<<B, 256, 0, st>>>(d_segs, d);
+        k_colstat_seg<1><<<sgrid, 256, 0, st>>>(d_segs, d, par);
+        k_finalize_seg<1><<<B, 256, 0, st>>>(d_segs, d);
+        k_normalize_seg<<<cdiv(total, 256), 256, 0, st>>>(d_segs, B, total, d, par);
+        NCVIS_LAUNCH_CHECK();
+    }
+    if (n_init_epochs % 2) {
+        k_copy_back_seg<<<cdiv(total, 256), 256, 0, st>>>(d_segs, B, total, d);
+        NCVIS_LAUNCH_CHECK();
+    }
+}
+
+static int stat_blocks(int64_t N) { return (int)std::min<int64_t>(1024, std::max<int64_t>(1, cdiv(N, 256))); }
 
 void init_embedding_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, float alpha,
                         int n_init_epochs, uint64_t seed, int64_t chains, float *dY, hipStream_t st) {
     if (N < 1 || d < 1 || d > 1024) throw HipError("[ncvis_amd] init: invalid sizes");
-    const int64_t Nd = N * d;
-    const int64_t T = chains < 1 ? 1 : chains;
-    k_init_random<<<cdiv(cdiv(Nd, 64), 256), 256, 0, st>>>(dY, Nd, T, seed);
-    NCVIS_LAUNCH_CHECK();
-    if (n_init_epochs <= 0) return;
-    DevBuf<float> y2((size_t)Nd);
-    const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, cdiv(N, 256)));
-    DevBuf<double> partial((size_t)nb * d);
+    const int nb = stat_blocks(N);
+    DevBuf<FitSeg> seg(1);
+    DevBuf<float> y2(n_init_epochs > 0 ? (size_t)N * d : 0);
+    DevBuf<double> part((size_t)nb * d);
     DevBuf<float> stats((size_t)2 * d);
-    float *Ys[2] = {dY, y2.p};
-    for (int ep = 0; ep < n_init_epochs; ++ep) {
-        const float *Yo = Ys[ep % 2];
-        float *Yn = Ys[(ep + 1) % 2];
-        switch (d) {
-            case 1: k_spmm_rows_d<1><<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, d_col, N, alpha, Yo, Yn); break;
-            case 2: k_spmm_rows_d<2><<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, d_col, N, alpha, Yo, Yn); break;
-            case 3: k_spmm_rows_d<3><<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, d_col, N, alpha, Yo, Yn); break;
-            case 4: k_spmm_rows_d<4><<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, d_col, N, alpha, Yo, Yn); break;
-            default: k_spmm_rows<<<cdiv(N, 256), 256, 0, st>>>(d_row_ptr, d_col, N, d, alpha, Yo, Yn); break;
-        }
-        NCVIS_LAUNCH_CHECK();
-        k_colstat<0><<<nb, 256, 0, st>>>(Yn, N, d, nullptr, partial.p);
-        k_finalize<0><<<1, 1024, 0, st>>>(partial.p, nb, d, N, stats.p);
-        k_colstat<1><<<nb, 256, 0, st>>>(Yn, N, d, stats.p, partial.p);
-        k_finalize<1><<<1, 1024, 0, st>>>(partial.p, nb, d, N, stats.p + d);
-        k_normalize<<<cdiv(Nd, 256), 256, 0, st>>>(Yn, N, d, stats.p, stats.p + d);
-        NCVIS_LAUNCH_CHECK();
-    }
-    if (n_init_epochs % 2)
-        NCVIS_HIP_CHECK(hipMemcpyAsync(dY, y2.p, sizeof(float) * Nd, hipMemcpyDeviceToDevice, st));
+    FitSeg v{};
+    v.Y = dY;
+    v.Y2 = y2.p;
+    v.row_ptr = d_row_ptr;
+    v.col = d_col;
+    v.N = N;
+    v.part = part.p;
+    v.stats = stats.p;
+    v.alpha_init = alpha;
+    v.nb = nb;
+    v.T = chains < 1 ? 1 : chains;
+    v.T_cap = v.T;
+    seg_set(seg.p, v, st);
+    SegTotals tot;
+    tot.rows = N;
+    tot.slots = v.T;
+    tot.max_nb = nb;
+    seg_init_embedding(seg.p, 1, tot, d, n_init_epochs, seed, st);
     NCVIS_HIP_CHECK(hipStreamSynchronize(st));
 }
 
@@ -293,39 +390,36 @@ __device__ inline void sgd_sample_rt(float *__restrict__ Y, int d, int64_t id, i
     }
 }
 
+
+// ------------------------------------------------------------------------ epoch bodies
 // chain_row[t] = row holding edge s0 of chain t (row_ptr[r] <= s0 < row_ptr[r+1])
-__global__ void k_chain_rows(const int64_t *__restrict__ row_ptr, int64_t N, int64_t E, int64_t T,
-                             int64_t *__restrict__ chain_row, U128 *__restrict__ rng, uint64_t seed) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T) return;
+__global__ void k_chain_rows_seg(const FitSeg *__restrict__ s, int B, int64_t slots, uint64_t seed) {
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gt >= slots) return;
+    const FitSeg &g = s[seg_find<1>(s, B, gt)];
+    const int64_t t = gt - g.chain_off;
+    if (t >= g.T) return;
     int64_t s0, e0;
-    static_chunk(E, T, t, s0, e0);
-    int64_t lo = 0, hi = N;  // largest r with row_ptr[r] <= s0
+    static_chunk(g.E, g.T, t, s0, e0);
+    int64_t lo = 0, hi = g.N;  // largest r with row_ptr[r] <= s0
     while (hi - lo > 1) {
-        int64_t mid = (lo + hi) / 2;
-        if (row_ptr[mid] <= s0)
+        const int64_t mid = (lo + hi) / 2;
+        if (g.row_ptr[mid] <= s0)
             lo = mid;
         else
             hi = mid;
     }
-    while (lo < N - 1 && row_ptr[lo + 1] <= s0) ++lo;
-    chain_row[t] = lo;
-    rng[t] = pcg_seed(seed + (uint64_t)t);
+    while (lo < g.N - 1 && g.row_ptr[lo + 1] <= s0) ++lo;
+    g.chain_row[t] = lo;
+    g.rng[t] = pcg_seed(seed + (uint64_t)t);
 }
 
+// One chain, sample by sample on memory (modes 0 and 1): the reference's thread loop
+// (ncvis.cpp:290-340). DIM 0: d only known at run time.
 template <int DIM>
-__global__ __launch_bounds__(256) void k_sgd_epoch(float *__restrict__ Y, int d_rt, const int64_t *__restrict__ row_ptr,
-                                                   const int32_t *__restrict__ col, int64_t E, int64_t T,
-                                                   const int64_t *__restrict__ chain_row, U128 *__restrict__ rng,
-                                                   const float *__restrict__ Qdev, float *__restrict__ qpart,
-                                                   SampleConst c, uint64_t N, int atomic_mode) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    int64_t s0, e0;
-    static_chunk(E, T, t, s0, e0);
-    float Qc = *Qdev;
-    U128 g = rng[t];
-    int64_t r = chain_row[t];
+__device__ __forceinline__ void chain_exact(float *__restrict__ Y, int d_rt, const int64_t *__restrict__ row_ptr,
+                                            const int32_t *__restrict__ col, int64_t s0, int64_t e0, int64_t r,
+                                            U128 &g, float &Qc, const SampleConst &c, uint64_t N, int atomic_mode) {
     int64_t rend = row_ptr[r + 1];
     for (int64_t e = s0; e < e0; ++e) {
         while (e >= rend) {
@@ -348,29 +442,18 @@ __global__ __launch_bounds__(256) void k_sgd_epoch(float *__restrict__ Y, int d_
                 sgd_sample_rt(Y, d_rt, id, other, j == 0, c, Qc, atomic_mode);
         }
     }
-    rng[t] = g;
-    qpart[t] = Qc;
 }
 
-// Many-chain (Hogwild) epoch. A chain keeps the row of its current source point in registers
-// and adds its own updates of that row there (the chain sees its own writes, as a thread of the
-// reference does), flushing them to Y with atomics when it moves to the next row; updates of
-// the other endpoint go to Y with atomics at once, so concurrent chains lose no update. The
-// negatives of an edge are drawn before their rows are read (the draw sequence does not depend
-// on Y), so PF rows are fetched together instead of one dependent load per sample.
+// One chain of the many-chain (Hogwild) mode. The chain keeps the row of its current source
+// point in registers and adds its own updates of that row there (it sees its own writes, as a
+// thread of the reference does), flushing them to Y with atomics when it moves to the next row;
+// updates of the other endpoint go to Y at once with atomics, so concurrent chains lose no
+// update. The negatives of an edge are drawn before their rows are read (the draw sequence does
+// not depend on Y), so PF rows are fetched together instead of one dependent load per sample.
 template <int DIM, int PF>
-__global__ __launch_bounds__(256) void k_sgd_epoch_hw(float *__restrict__ Y, const int64_t *__restrict__ row_ptr,
-                                                      const int32_t *__restrict__ col, int64_t E, int64_t T,
-                                                      const int64_t *__restrict__ chain_row, U128 *__restrict__ rng,
-                                                      const float *__restrict__ Qdev, float *__restrict__ qpart,
-                                                      SampleConst c, uint64_t N, int plain_far) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    int64_t s0, e0;
-    static_chunk(E, T, t, s0, e0);
-    float Qc = *Qdev;
-    U128 g = rng[t];
-    int64_t r = chain_row[t];
+__device__ __forceinline__ void chain_hw(float *__restrict__ Y, const int64_t *__restrict__ row_ptr,
+                                         const int32_t *__restrict__ col, int64_t s0, int64_t e0, int64_t r,
+                                         U128 &g, float &Qc, const SampleConst &c, uint64_t N, int plain_far) {
     int64_t rend = row_ptr[r + 1];
     float base[DIM], acc[DIM];
 #pragma unroll
@@ -460,15 +543,47 @@ __global__ __launch_bounds__(256) void k_sgd_epoch_hw(float *__restrict__ Y, con
 #pragma unroll
     for (int k = 0; k < DIM; ++k)
         if (acc[k] != 0.f) __hip_atomic_fetch_add(&Y[r * DIM + k], acc[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    rng[t] = g;
-    qpart[t] = Qc;
 }
 
-// One block of 1024 threads; each thread keeps four loads in flight so the 1 MB of chain copies at
-// 2^18 chains is read at bandwidth instead of one dependent load per thread per step.
-__global__ __launch_bounds__(1024) void k_q_reduce(const float *__restrict__ qpart, int64_t T,
-                                                   float *__restrict__ Qdev, float *__restrict__ trace, int epoch) {
+// One epoch of every chain of every data set: thread = chain slot (slots past a data set's T
+// exit). DIM 1..4: d at compile time (mode 2 available); DIM 0: any d (modes 0 and 1).
+template <int DIM, int PF>
+__global__ __launch_bounds__(256) void k_sgd_epoch_seg(const FitSeg *__restrict__ s, int B, int64_t slots,
+                                                       SampleConst c, int d_rt, int plain_far) {
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gt >= slots) return;
+    const FitSeg &g = s[seg_find<1>(s, B, gt)];
+    const int64_t t = gt - g.chain_off;
+    const int64_t T = g.T;
+    if (t >= T) return;
+    int64_t s0, e0;
+    static_chunk(g.E, T, t, s0, e0);
+    float Qc = g.qdev[0];
+    U128 rng = g.rng[t];
+    const int64_t r = g.chain_row[t];
+    const uint64_t N = (uint64_t)g.N;
+    const int mode = g.mode;
+    SampleConst ce = c;
+    ce.fast_math = 0;  // the exact modes keep the reference's rounding whatever the knob says
+    if constexpr (DIM >= 1 && DIM <= 4) {
+        if (mode == 2)
+            chain_hw<DIM, PF>(g.Y, g.row_ptr, g.col, s0, e0, r, rng, Qc, c, N, plain_far);
+        else
+            chain_exact<DIM>(g.Y, DIM, g.row_ptr, g.col, s0, e0, r, rng, Qc, ce, N, mode);
+    } else {
+        chain_exact<0>(g.Y, d_rt, g.row_ptr, g.col, s0, e0, r, rng, Qc, ce, N, mode != 0);
+    }
+    g.rng[t] = rng;
+    g.qpart[t] = Qc;
+}
+
+// One 1024-thread block per data set; each thread keeps four loads in flight so 2^17 chain
+// copies are read at bandwidth instead of one dependent load per thread per step.
+__global__ __launch_bounds__(1024) void k_q_reduce_seg(const FitSeg *__restrict__ s, int epoch) {
     __shared__ double red[1024 / 64];
+    const FitSeg &g = s[blockIdx.x];
+    const int64_t T = g.T;
+    const float *__restrict__ qpart = g.qpart;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int64_t t = threadIdx.x;
     for (; t + 3 * 1024 < T; t += 4 * 1024) {
@@ -478,58 +593,71 @@ __global__ __launch_bounds__(1024) void k_q_reduce(const float *__restrict__ qpa
         s3 += (double)qpart[t + 3 * 1024];
     }
     for (; t < T; t += 1024) s0 += (double)qpart[t];
-    double s = (s0 + s1) + (s2 + s3);
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    double sum = (s0 + s1) + (s2 + s3);
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
     __syncthreads();
     if (threadIdx.x == 0) {
         double tot = 0.0;
         for (int w = 0; w < 1024 / 64; ++w) tot += red[w];
-        float q = (T == 1) ? qpart[0] : (float)(tot / (double)T);
-        *Qdev = q;
-        trace[epoch] = q;
+        const float q = (T == 1) ? qpart[0] : (float)(tot / (double)T);
+        g.qdev[0] = q;
+        g.qdev[1 + epoch] = q;
     }
 }
 
-template <int PF>
-static void launch_hw(int d, dim3 grid, float *Y, const int64_t *row_ptr, const int32_t *col, int64_t E, int64_t T,
-                      const int64_t *chain_row, U128 *rng, const float *Qdev, float *qpart, const SampleConst &c,
-                      uint64_t N, int plain_far, hipStream_t st) {
-    switch (d) {
-        case 1: k_sgd_epoch_hw<1, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, plain_far); break;
-        case 2: k_sgd_epoch_hw<2, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, plain_far); break;
-        case 3: k_sgd_epoch_hw<3, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, plain_far); break;
-        default: k_sgd_epoch_hw<4, PF><<<grid, 256, 0, st>>>(Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, plain_far); break;
+__global__ void k_seg_q0(const FitSeg *__restrict__ s, int B, float Q0) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B) s[b].qdev[0] = Q0;
+}
+
+template <int DIM>
+static void launch_epoch_seg(int PF, dim3 grid, const FitSeg *segs, int B, int64_t slots, const SampleConst &c,
+                             int d, int plain_far, hipStream_t st) {
+    switch (PF) {
+        case 2: k_sgd_epoch_seg<DIM, 2><<<grid, 256, 0, st>>>(segs, B, slots, c, d, plain_far); break;
+        case 4: k_sgd_epoch_seg<DIM, 4><<<grid, 256, 0, st>>>(segs, B, slots, c, d, plain_far); break;
+        case 8: k_sgd_epoch_seg<DIM, 8><<<grid, 256, 0, st>>>(segs, B, slots, c, d, plain_far); break;
+        default: k_sgd_epoch_seg<DIM, 1><<<grid, 256, 0, st>>>(segs, B, slots, c, d, plain_far); break;
     }
 }
 
-static void launch_epoch(int d, dim3 grid, float *Y, const int64_t *row_ptr, const int32_t *col, int64_t E,
-                         int64_t T, const int64_t *chain_row, U128 *rng, const float *Qdev, float *qpart,
-                         const SampleConst &c, uint64_t N, int atomic_mode, hipStream_t st) {
-    // the many-chain kernel updates coordinate pairs with 64-bit compare-and-swap, which needs
-    // 8-byte aligned rows; a Y that is only 4-byte aligned takes the float-atomic kernel
-    const bool pairs_aligned = (d % 2 != 0) || ((uintptr_t)Y % 8 == 0);
-    if (atomic_mode && d >= 1 && d <= 4 && pairs_aligned) {
-        const Tuning &tu = tuning();
-        SampleConst cc = c;
-        cc.fast_math = tu.sgd_fast_math;
-        switch (tu.sgd_prefetch) {
-            case 1: launch_hw<1>(d, grid, Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, cc, N, tu.sgd_plain_far, st); break;
-            case 2: launch_hw<2>(d, grid, Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, cc, N, tu.sgd_plain_far, st); break;
-            case 8: launch_hw<8>(d, grid, Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, cc, N, tu.sgd_plain_far, st); break;
-            default: launch_hw<4>(d, grid, Y, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, cc, N, tu.sgd_plain_far, st); break;
+void seg_optimize(const FitSeg *d_segs, int B, const SegTotals &tot, int d, float Q0, const SgdParams &p,
+                  hipStream_t st, hipEvent_t *epoch_events) {
+    if (tot.slots <= 0) return;
+    const Tuning &tu = tuning();
+    k_seg_q0<<<cdiv(B, 64), 64, 0, st>>>(d_segs, B, Q0);
+    k_chain_rows_seg<<<cdiv(tot.slots, 256), 256, 0, st>>>(d_segs, B, tot.slots, p.seed);
+    NCVIS_LAUNCH_CHECK();
+    const dim3 grid((unsigned)cdiv(tot.slots, 256));
+    for (int epoch = 0; epoch < p.n_epochs; ++epoch) {
+        SampleConst c;
+        c.a = p.a;
+        c.b = p.b;
+        c.alpha_Q = p.alpha_Q;
+        const float fe = (float)epoch / (float)p.n_epochs;
+        c.step = p.alpha * (1.f - fe * fe);
+        c.noise = (uint32_t)p.n_noise[epoch];
+        c.noise_f = (float)p.n_noise[epoch];
+        c.fast_math = tu.sgd_fast_math;
+        if (epoch_events) NCVIS_HIP_CHECK(hipEventRecord(epoch_events[2 * epoch], st));
+        switch (d) {
+            case 1: launch_epoch_seg<1>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, st); break;
+            case 2: launch_epoch_seg<2>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, st); break;
+            case 3: launch_epoch_seg<3>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, st); break;
+            case 4: launch_epoch_seg<4>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, st); break;
+            default:
+                k_sgd_epoch_seg<0, 1><<<grid, 256, 0, st>>>(d_segs, B, tot.slots, c, d, tu.sgd_plain_far);
+                break;
         }
-        return;
-    }
-    switch (d) {
-        case 1: k_sgd_epoch<1><<<grid, 256, 0, st>>>(Y, d, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, atomic_mode); break;
-        case 2: k_sgd_epoch<2><<<grid, 256, 0, st>>>(Y, d, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, atomic_mode); break;
-        case 3: k_sgd_epoch<3><<<grid, 256, 0, st>>>(Y, d, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, atomic_mode); break;
-        case 4: k_sgd_epoch<4><<<grid, 256, 0, st>>>(Y, d, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, atomic_mode); break;
-        default: k_sgd_epoch<0><<<grid, 256, 0, st>>>(Y, d, row_ptr, col, E, T, chain_row, rng, Qdev, qpart, c, N, atomic_mode); break;
+        NCVIS_LAUNCH_CHECK();
+        if (epoch_events) NCVIS_HIP_CHECK(hipEventRecord(epoch_events[2 * epoch + 1], st));
+        k_q_reduce_seg<<<B, 1024, 0, st>>>(d_segs, epoch);
+        NCVIS_LAUNCH_CHECK();
     }
 }
 
+// One data set with host-known E and chain count (the stage entry points).
 void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t E, float *dY, float *Q,
                   const SgdParams &p, hipStream_t st, float *q_trace, double *epoch_kernel_ms,
                   const SgdWorkspace *ws) {
@@ -538,6 +666,7 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
     DevBuf<int64_t> own_chain_row;
     DevBuf<U128> own_rng;
     DevBuf<float> own_qpart, own_qdev;
+    DevBuf<FitSeg> own_seg;
     SgdWorkspace w;
     if (ws) {
         w = *ws;
@@ -546,13 +675,27 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
         own_rng.alloc((size_t)T);
         own_qpart.alloc((size_t)T);
         own_qdev.alloc(1 + (size_t)std::max(p.n_epochs, 1));
-        w = SgdWorkspace{own_chain_row.p, own_rng.p, own_qpart.p, own_qdev.p};
+        own_seg.alloc(1);
+        w = SgdWorkspace{own_chain_row.p, own_rng.p, own_qpart.p, own_qdev.p, own_seg.p};
     }
-    k_chain_rows<<<cdiv(T, 256), 256, 0, st>>>(d_row_ptr, N, E, T, w.chain_row, w.rng, p.seed);
-    NCVIS_LAUNCH_CHECK();
-    NCVIS_HIP_CHECK(hipMemcpyAsync(w.qdev, Q, sizeof(float), hipMemcpyHostToDevice, st));
-    dim3 grid((unsigned)cdiv(T, 256));
-    // Optional per-epoch kernel timing: one event pair around each epoch kernel on `st`.
+    FitSeg v{};
+    v.Y = dY;
+    v.row_ptr = d_row_ptr;
+    v.col = d_col;
+    v.N = N;
+    v.T_cap = T;
+    v.chain_row = w.chain_row;
+    v.rng = w.rng;
+    v.qpart = w.qpart;
+    v.qdev = w.qdev;
+    v.pairs_ok = ((uintptr_t)dY % 8 == 0) ? 1 : 0;
+    v.E = E;
+    v.T = T;
+    v.mode = p.atomic_updates ? ((d <= 4 && (d % 2 != 0 || v.pairs_ok)) ? 2 : 1) : 0;
+    seg_set(w.seg, v, st);
+    SegTotals tot;
+    tot.rows = N;
+    tot.slots = T;
     std::vector<hipEvent_t> ev;
     struct EvGuard {
         std::vector<hipEvent_t> &e;
@@ -567,38 +710,23 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
             NCVIS_HIP_CHECK(hipEventCreate(&x));
         }
     }
-    for (int epoch = 0; epoch < p.n_epochs; ++epoch) {
-        SampleConst c;
-        c.a = p.a;
-        c.b = p.b;
-        c.alpha_Q = p.alpha_Q;
-        float fe = (float)epoch / (float)p.n_epochs;
-        c.step = p.alpha * (1.f - fe * fe);
-        c.noise = (uint32_t)p.n_noise[epoch];
-        c.noise_f = (float)p.n_noise[epoch];
-        c.fast_math = 0;
-        if (epoch_kernel_ms) NCVIS_HIP_CHECK(hipEventRecord(ev[2 * epoch], st));
-        launch_epoch(d, grid, dY, d_row_ptr, d_col, E, T, w.chain_row, w.rng, w.qdev, w.qpart, c, (uint64_t)N,
-                     p.atomic_updates, st);
-        NCVIS_LAUNCH_CHECK();
-        if (epoch_kernel_ms) NCVIS_HIP_CHECK(hipEventRecord(ev[2 * epoch + 1], st));
-        k_q_reduce<<<1, 1024, 0, st>>>(w.qpart, T, w.qdev, w.qdev + 1, epoch);
-        NCVIS_LAUNCH_CHECK();
-    }
+    seg_optimize(w.seg, 1, tot, d, *Q, p, st, epoch_kernel_ms ? ev.data() : nullptr);
     NCVIS_HIP_CHECK(hipMemcpyAsync(Q, w.qdev, sizeof(float), hipMemcpyDeviceToHost, st));
     if (q_trace && p.n_epochs > 0)
         NCVIS_HIP_CHECK(
             hipMemcpyAsync(q_trace, w.qdev + 1, sizeof(float) * p.n_epochs, hipMemcpyDeviceToHost, st));
     NCVIS_HIP_CHECK(hipStreamSynchronize(st));
-    if (epoch_kernel_ms) {
-        double tot = 0.0;
-        for (int epoch = 0; epoch < p.n_epochs; ++epoch) {
-            float ms = 0.f;
-            NCVIS_HIP_CHECK(hipEventElapsedTime(&ms, ev[2 * epoch], ev[2 * epoch + 1]));
-            tot += ms;
-        }
-        *epoch_kernel_ms = tot;
+    if (epoch_kernel_ms) *epoch_kernel_ms = epoch_time_ms(ev.data(), p.n_epochs);
+}
+
+double epoch_time_ms(const hipEvent_t *ev, int n_epochs) {
+    double tot = 0.0;
+    for (int epoch = 0; epoch < n_epochs; ++epoch) {
+        float ms = 0.f;
+        NCVIS_HIP_CHECK(hipEventElapsedTime(&ms, ev[2 * epoch], ev[2 * epoch + 1]));
+        tot += ms;
     }
+    return tot;
 }
 
 // ------------------------------------------------------------------------ graph checks

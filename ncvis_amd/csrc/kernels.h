@@ -5,6 +5,10 @@
 
 namespace ncvis_hip {
 
+// error flags a queued fit leaves in its device error word (read once at the end)
+constexpr int kErrKnnCandidates = 1;  // a row had fewer than k+1 finite candidates
+constexpr int kErrSymRange = 2;       // a kNN index outside [0, N)
+
 // Process-wide tuning knobs (ncvis_tune in the C ABI). Defaults are the measured best on MI355X;
 // none of them changes the result of a single-chain (n_threads = 1) fit.
 struct Tuning {
@@ -31,6 +35,16 @@ struct KnnStats {
 };
 void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32_t *d_inds, float *d_dists,
              hipStream_t st, KnnStats *stats = nullptr);
+// Events recorded around the candidate tiles and the rerank (optional).
+struct KnnEvents {
+    hipEvent_t tiles_begin, tiles_end, rerank_end;
+};
+// The same search queued on `st` with no host synchronisation and no allocation: ws holds
+// knn_workspace_bytes(N, D, k, dist, prec); a row left with fewer than k+1 finite candidates sets
+// kErrKnnCandidates in *d_err (which the caller zeroes).
+size_t knn_workspace_bytes(int64_t N, int D, int k, int dist, int prec);
+void knn_enqueue(const float *dX, int64_t N, int D, int k, int dist, int prec, int32_t *d_inds, float *d_dists,
+                 void *ws, int *d_err, hipStream_t st, const KnnEvents *ev = nullptr);
 
 // Candidate-tile launches on the packed matrix Xh [Npad, 32*DK] and offsets cn [Npad], writing the
 // two lane half-lists of KL ids per query to cand [N, 2*KL] (one translation unit each).
@@ -54,9 +68,15 @@ void knn_tiles_wide_f32(int KL, int DK, int64_t N, int64_t Npad, const void *Xh,
 
 // ---- symmetrize (symmetrize.hip) ---------------------------------------------------------
 // KNNTable::symmetrize at one thread, bit-exact, then flattened to CSR (= build_edges order).
-// d_col / d_dist must hold 2*N*k entries; d_row_ptr N+1. Returns E.
+// d_col / d_dist must hold 2*N*k entries; d_row_ptr N+1. Synchronous; returns E.
 int64_t symmetrize_run(const int32_t *d_inds, const float *d_dists, int64_t N, int k, int64_t *d_row_ptr,
                        int32_t *d_col, float *d_dist, hipStream_t st);
+// The same queued on `st` with no host synchronisation and no allocation: ws holds
+// symmetrize_workspace_bytes(N, k); E ends up in d_row_ptr[N]; an index outside [0, N) sets
+// kErrSymRange in *d_err (which the caller zeroes) and is left out of the incoming lists.
+size_t symmetrize_workspace_bytes(int64_t N, int k);
+void symmetrize_enqueue(const int32_t *d_inds, const float *d_dists, int64_t N, int k, int64_t *d_row_ptr,
+                        int32_t *d_col, float *d_dist, void *ws, int *d_err, hipStream_t st);
 
 // ---- embedding (embed.hip) ---------------------------------------------------------------
 struct SgdParams {
@@ -68,21 +88,86 @@ struct SgdParams {
     int atomic_updates;       // 1: Y updates with atomics (Hogwild without lost updates)
 };
 
+// Chain count of a data set with E edges (the reference's thread count on the GPU): `fixed` if
+// positive, else E / edges_per_chain within [min(E / 16, min_chains), max_chains], at least 1.
+// atomic_opt: -1 atomics exactly when more than one chain runs, 0 never, 1 always.
+struct ChainRule {
+    int64_t fixed;
+    int64_t edges_per_chain, min_chains, max_chains;
+    int atomic_opt;
+};
+__host__ __device__ inline int64_t chain_rule(int64_t E, const ChainRule &r) {
+    if (r.fixed > 0) return r.fixed;
+    int64_t c = E / (r.edges_per_chain > 0 ? r.edges_per_chain : 1);
+    const int64_t lo = (E / 16 < r.min_chains) ? E / 16 : r.min_chains;
+    if (c < lo) c = lo;
+    if (c > r.max_chains) c = r.max_chains;
+    return c < 1 ? 1 : c;
+}
+
+// One data set of a fit, in device memory. The init and epoch kernels walk the concatenation of
+// B data sets (rows at row_off, chain slots at chain_off, both ascending).
+struct FitSeg {
+    float *Y;                  // [N, d] embedding
+    float *Y2;                 // [N, d] init double buffer (null without init epochs)
+    const int64_t *row_ptr;    // [N + 1]
+    const int32_t *col;        // [E]
+    int64_t N;
+    int64_t row_off;           // first row of this data set in the concatenated rows
+    int64_t chain_off;         // first chain slot of this data set
+    int64_t T_cap;             // chain slots reserved (T never exceeds it)
+    int64_t *chain_row;        // [T_cap] row of each chain's first edge
+    U128 *rng;                 // [T_cap] pcg64 state of each chain
+    float *qpart;              // [T_cap] each chain's Q copy
+    float *qdev;               // [1 + n_epochs]: Q, then Q after each epoch
+    double *part;              // [nb * d] init statistics partial sums
+    float *stats;              // [2 d] init mean and sigma
+    float alpha_init;          // propagation weight 1 / k (ncvis.cpp:418)
+    int nb;                    // blocks of the init statistics (min(1024, ceil(N / 256)))
+    int pairs_ok;              // Y is 8-byte aligned (64-bit compare-and-swap on pairs)
+    int64_t E, T;              // edges and chains (k_seg_dims, or the host for stage calls)
+    int mode;                  // 0 exact plain stores, 1 exact with float atomics, 2 many-chain
+};
+
+struct SegTotals {
+    int64_t rows = 0;   // sum of N
+    int64_t slots = 0;  // sum of T_cap
+    int max_nb = 1;     // largest nb
+};
+
+// Writes one descriptor from the host value (a kernel argument: no copy, capturable).
+void seg_set(FitSeg *d_seg, const FitSeg &v, hipStream_t st);
+// E, T and mode of every data set from its graph on the device.
+void seg_dims(FitSeg *d_segs, int B, const ChainRule &rule, int d, hipStream_t st);
+// Random layout + n_init_epochs propagation epochs of every data set (T must be set).
+void seg_init_embedding(const FitSeg *d_segs, int B, const SegTotals &tot, int d, int n_init_epochs, uint64_t seed,
+                        hipStream_t st);
+// All SGD epochs of every data set from Q0 (E, T, mode must be set); no host synchronisation.
+// epoch_events (optional, 2 * n_epochs events) bracket each epoch kernel.
+void seg_optimize(const FitSeg *d_segs, int B, const SegTotals &tot, int d, float Q0, const SgdParams &p,
+                  hipStream_t st, hipEvent_t *epoch_events);
+// summed time of the epoch kernels between event pairs
+double epoch_time_ms(const hipEvent_t *ev, int n_epochs);
+
+// One data set, synchronous (the stage entry points): random layout + propagation epochs with
+// `chains` threads' static schedule of the random draws.
 void init_embedding_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, float alpha,
                         int n_init_epochs, uint64_t seed, int64_t chains, float *dY, hipStream_t st);
 
 // Device scratch of the epoch loop, for callers that bring their own memory: chain_row [T],
-// rng [T] (16-byte aligned), qpart [T], qdev [1 + n_epochs].
+// rng [T] (16-byte aligned), qpart [T], qdev [1 + n_epochs], seg [1].
 struct SgdWorkspace {
     int64_t *chain_row;
     U128 *rng;
     float *qpart;
     float *qdev;
+    FitSeg *seg;
 };
 
-// Runs all epochs; Q (host in/out). `q_trace` (optional, host [n_epochs]) receives Q per epoch;
-// `epoch_kernel_ms` (optional, host) the summed event time of the epoch kernels alone. `ws`
-// (optional) supplies the device scratch; without it the scratch is allocated here.
+// One data set with host-known E and chain count, synchronous; Q (host in/out). `q_trace`
+// (optional, host [n_epochs]) receives Q per epoch; `epoch_kernel_ms` (optional, host) the summed
+// event time of the epoch kernels alone. `ws` (optional) supplies the device scratch; without it
+// the scratch is allocated here.
 void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t E, float *dY,
                   float *Q, const SgdParams &p, hipStream_t st, float *q_trace = nullptr,
                   double *epoch_kernel_ms = nullptr, const SgdWorkspace *ws = nullptr);

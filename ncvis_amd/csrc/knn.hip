@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ Xp
         rank += key_less(dj, ij, j, dv, ci, c) ? 1 : 0;
     }
     if (valid && rank >= 1 && rank <= k) {
-        if (ci == INT_MAX) atomicExch(err, 1);
+        if (ci == INT_MAX) atomicOr(err, kErrKnnCandidates);
         inds[q * k + (k - rank)] = (ci == INT_MAX) ? 0 : ci;
         dists[q * k + (k - rank)] = dv;
     }
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void k_knn_rerank128(const float *__restrict__
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
         if (valid && rank[u] >= 1 && rank[u] <= k) {
-            if (ci[u] == INT_MAX) atomicExch(err, 1);
+            if (ci[u] == INT_MAX) atomicOr(err, kErrKnnCandidates);
             inds[q * k + (k - rank[u])] = (ci[u] == INT_MAX) ? 0 : ci[u];
             dists[q * k + (k - rank[u])] = dv[u];
         }
@@ -479,129 +479,183 @@ __global__ __launch_bounds__(256) void k_knn_generic(const float *__restrict__ X
     }
 }
 
+// The shape decisions of one search and the carving of its workspace (256-byte boundaries).
+struct KnnLayout {
+    bool tiled, wide, wide32;
+    int DK, Dpad, QB, KL, lists, C, esz, nb;
+    int64_t Npad;
+    size_t xp, part, mean, cs, xh, cn, cand, hd, hi, total;
+};
+
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+static KnnLayout knn_layout(int64_t N, int D, int k, int dist, int prec) {
+    KnnLayout L{};
+    // rows up to 256 elements: 32x32 MFMA tiles in every precision (k + 1 <= 64); rows of
+    // 257..512: one query set per wave in the 16-bit GEMMs, 16x16x4 tiles in FP32 (k + 1 <= 32);
+    // wider rows or longer lists take the generic exact scan
+    L.wide = (D > 256) && (D <= 512);
+    L.wide32 = L.wide && prec == PREC_FP32;
+    L.KL = (k + 1 <= 16) ? 16 : (k + 1 <= 32) ? 32 : 64;
+    L.tiled = (D <= 256 && k + 1 <= 64) || (L.wide && k + 1 <= 32);
+    L.DK = D <= 32 ? 1 : D <= 64 ? 2 : D <= 128 ? 4 : D <= 256 ? 8 : D <= 320 ? 10 : D <= 384 ? 12 : 16;
+    L.Dpad = 32 * L.DK;
+    L.QB = (L.wide || L.KL >= 32 || prec == PREC_FP32) ? 1 : 2;
+    const int64_t qtile = L.wide32 ? 64 : 4 * 32 * L.QB;
+    L.Npad = ((N + qtile - 1) / qtile) * qtile;
+    L.lists = L.wide32 ? 4 : 2;  // candidate lists per query
+    L.C = L.lists * L.KL;        // candidates per query for the rerank: 32, 64 or 128
+    L.esz = (prec == PREC_FP32) ? 4 : 2;
+    L.nb = (int)std::min<int64_t>(512, std::max<int64_t>(1, cdiv(N, 256)));
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align256(o + bytes);
+        return at;
+    };
+    L.xp = take((dist == DIST_COSINE || dist == DIST_CORRELATION) ? sizeof(float) * N * D : 0);
+    if (L.tiled) {
+        L.part = take(sizeof(double) * L.nb * D);
+        L.mean = take(sizeof(double) * D);
+        L.cs = take(sizeof(float) * (D + 1));
+        L.xh = take((size_t)L.Npad * L.Dpad * L.esz);
+        L.cn = take(sizeof(float) * L.Npad);
+        L.cand = take(sizeof(int32_t) * N * L.C);
+    } else {
+        L.hd = take(sizeof(float) * N * (k + 1));
+        L.hi = take(sizeof(int) * N * (k + 1));
+    }
+    L.total = std::max<size_t>(o, 256);
+    return L;
+}
+
+size_t knn_workspace_bytes(int64_t N, int D, int k, int dist, int prec) {
+    return knn_layout(N, D, k, dist, prec).total;
+}
+
 // cs [D + 1]: per-column offsets (0 unless centring) and the scale, all on the device.
-static void gemm_input_transform(const float *Xp, int64_t N, int D, int center, float *cs, hipStream_t st) {
-    const int nb = (int)std::min<int64_t>(512, std::max<int64_t>(1, cdiv(N, 256)));
+static void gemm_input_transform(const float *Xp, int64_t N, int D, int center, int nb, double *part, double *mean,
+                                 float *cs, hipStream_t st) {
     const dim3 grid((unsigned)cdiv(D, 64), (unsigned)nb), block(64, 4);
-    DevBuf<double> part((size_t)nb * D);
-    DevBuf<double> mean;
     if (center) {
-        mean.alloc((size_t)D);
-        k_colscan<0><<<grid, block, 0, st>>>(Xp, N, D, nullptr, part.p);
-        k_colmean<<<cdiv(D, 256), 256, 0, st>>>(part.p, nb, N, D, mean.p);
+        k_colscan<0><<<grid, block, 0, st>>>(Xp, N, D, nullptr, part);
+        k_colmean<<<cdiv(D, 256), 256, 0, st>>>(part, nb, N, D, mean);
         NCVIS_LAUNCH_CHECK();
     }
-    k_colscan<1><<<grid, block, 0, st>>>(Xp, N, D, mean.p, part.p);
-    k_pack_params<<<1, 256, 0, st>>>(mean.p, part.p, nb, D, cs);
+    k_colscan<1><<<grid, block, 0, st>>>(Xp, N, D, center ? mean : nullptr, part);
+    k_pack_params<<<1, 256, 0, st>>>(center ? mean : nullptr, part, nb, D, cs);
     NCVIS_LAUNCH_CHECK();
 }
+
+void knn_enqueue(const float *dX, int64_t N, int D, int k, int dist, int prec, int32_t *d_inds, float *d_dists,
+                 void *ws, int *d_err, hipStream_t st, const KnnEvents *ev) {
+    if (N < 2 || D < 1 || k < 1 || k > N - 1) throw HipError("[ncvis_amd] knn: invalid sizes");
+    if (N > (int64_t)INT_MAX - 512) throw HipError("[ncvis_amd] knn: more than 2^31 points");
+    if (prec < PREC_FP32 || prec > PREC_FP16) throw HipError("[ncvis_amd] knn: unknown precision");
+    const KnnLayout L = knn_layout(N, D, k, dist, prec);
+    unsigned char *w = (unsigned char *)ws;
+    const int is_l2 = (dist == DIST_L2);
+    const float *Xp = dX;
+    if (dist == DIST_COSINE || dist == DIST_CORRELATION) {
+        float *xp = (float *)(w + L.xp);
+        k_preprocess<<<cdiv(N, 256), 256, 0, st>>>(dX, N, D, dist, xp);
+        NCVIS_LAUNCH_CHECK();
+        Xp = xp;
+    }
+    if (L.tiled) {
+        const int DK = L.DK, Dpad = L.Dpad, KL = L.KL;
+        const int64_t Npad = L.Npad;
+        void *xh = w + L.xh;
+        float *cn = (float *)(w + L.cn);
+        float *cs = (float *)(w + L.cs);
+        int32_t *cand = (int32_t *)(w + L.cand);
+        gemm_input_transform(Xp, N, D, is_l2, L.nb, (double *)(w + L.part), (double *)(w + L.mean), cs, st);
+        switch (prec) {
+            case PREC_FP32:
+                if (L.wide32)
+                    k_pack<PREC_FP32, 4><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs, xh, cn);
+                else
+                    k_pack<PREC_FP32, 2><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs, xh, cn);
+                break;
+            case PREC_BF16:
+                k_pack<PREC_BF16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs, xh, cn);
+                break;
+            default:
+                k_pack<PREC_FP16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs, xh, cn);
+                break;
+        }
+        NCVIS_LAUNCH_CHECK();
+        const float beta = is_l2 ? -2.0f : -1.0f;
+        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->tiles_begin, st));
+        if (L.wide32)
+            knn_tiles_wide_f32(KL, DK, N, Npad, xh, cn, beta, cand, st);
+        else if (L.wide)
+            knn_tiles_wide(prec, KL, DK, N, Npad, xh, cn, beta, cand, st);
+        else if (KL == 16)
+            knn_tiles16(prec, DK, N, Npad, xh, cn, beta, cand, st);
+        else if (prec == PREC_FP32 && tuning().knn_dma != 0)
+            knn_tiles_q1_f32dma(KL, DK, N, Npad, xh, cn, beta, cand, st);
+        else if (prec == PREC_FP32)
+            knn_tiles_q1_f32(KL, DK, N, Npad, xh, cn, beta, cand, st);
+        else if (prec == PREC_BF16)
+            knn_tiles_q1_bf16(KL, DK, N, Npad, xh, cn, beta, cand, st);
+        else
+            knn_tiles_q1_fp16(KL, DK, N, Npad, xh, cn, beta, cand, st);
+        NCVIS_LAUNCH_CHECK();
+        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->tiles_end, st));
+        const int vec4 = (D % 4 == 0) && ((uintptr_t)Xp % 16 == 0);
+        if (L.C == 32)
+            k_knn_rerank<32><<<cdiv(N, 8), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand, k, d_inds, d_dists, d_err);
+        else if (L.C == 64)
+            k_knn_rerank<64><<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand, k, d_inds, d_dists, d_err);
+        else
+            k_knn_rerank128<<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand, k, d_inds, d_dists, d_err);
+        NCVIS_LAUNCH_CHECK();
+        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->rerank_end, st));
+    } else {
+        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->tiles_begin, st));
+        k_knn_generic<<<cdiv(N, 256), 256, 0, st>>>(Xp, N, D, is_l2, k, (float *)(w + L.hd), (int *)(w + L.hi),
+                                                    d_inds, d_dists);
+        NCVIS_LAUNCH_CHECK();
+        if (ev) {
+            NCVIS_HIP_CHECK(hipEventRecord(ev->tiles_end, st));
+            NCVIS_HIP_CHECK(hipEventRecord(ev->rerank_end, st));
+        }
+    }
+}
+
+__global__ void k_zero_int(int *p) { *p = 0; }
 
 void knn_run(const float *dX, int64_t N, int D, int k, int dist, int prec, int32_t *d_inds, float *d_dists,
              hipStream_t st, KnnStats *stats) {
     if (N < 2 || D < 1 || k < 1 || k > N - 1) throw HipError("[ncvis_amd] knn: invalid sizes");
-    if (N > (int64_t)INT_MAX - 512) throw HipError("[ncvis_amd] knn: more than 2^31 points");
-    const int is_l2 = (dist == DIST_L2);
-    DevBuf<float> xp;
-    const float *Xp = dX;
-    if (dist == DIST_COSINE || dist == DIST_CORRELATION) {
-        xp.alloc((size_t)N * D);
-        k_preprocess<<<cdiv(N, 256), 256, 0, st>>>(dX, N, D, dist, xp.p);
-        NCVIS_LAUNCH_CHECK();
-        Xp = xp.p;
-    }
+    DevBuf<unsigned char> ws(knn_workspace_bytes(N, D, k, dist, prec));
     DevBuf<int> err(1);
-    NCVIS_HIP_CHECK(hipMemsetAsync(err.p, 0, sizeof(int), st));
-    hipEvent_t e0, e1, e2;
-    NCVIS_HIP_CHECK(hipEventCreate(&e0));
-    NCVIS_HIP_CHECK(hipEventCreate(&e1));
-    NCVIS_HIP_CHECK(hipEventCreate(&e2));
-    // rows up to 256 elements: 32x32 MFMA tiles in every precision (k + 1 <= 64); rows of
-    // 257..512: one query set per wave in the 16-bit GEMMs, 16x16x4 tiles in FP32 (k + 1 <= 32);
-    // wider rows or longer lists take the generic exact scan
-    const bool wide = (D > 256) && (D <= 512);
-    const bool wide32 = wide && prec == PREC_FP32;
-    const int KL = (k + 1 <= 16) ? 16 : (k + 1 <= 32) ? 32 : 64;
-    const bool tiled = (D <= 256 && k + 1 <= 64) || (wide && k + 1 <= 32);
-    if (tiled) {
-        const int DK = D <= 32 ? 1 : D <= 64 ? 2 : D <= 128 ? 4 : D <= 256 ? 8 : D <= 320 ? 10 : D <= 384 ? 12 : 16;
-        const int Dpad = 32 * DK;
-        const int QB = (wide || KL >= 32 || prec == PREC_FP32) ? 1 : 2;
-        const int64_t qtile = wide32 ? 64 : 4 * 32 * QB;
-        const int lists = wide32 ? 4 : 2;  // candidate lists per query
-        const int64_t Npad = ((N + qtile - 1) / qtile) * qtile;
-        const int esz = (prec == PREC_FP32) ? 4 : 2;
-        DevBuf<unsigned char> xh((size_t)Npad * Dpad * esz);
-        DevBuf<float> cn((size_t)Npad);
-        DevBuf<float> cs((size_t)D + 1);
-        gemm_input_transform(Xp, N, D, is_l2, cs.p, st);
-        switch (prec) {
-            case PREC_FP32:
-                if (wide32)
-                    k_pack<PREC_FP32, 4><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs.p, xh.p, cn.p);
-                else
-                    k_pack<PREC_FP32, 2><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs.p, xh.p, cn.p);
-                break;
-            case PREC_BF16:
-                k_pack<PREC_BF16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs.p, xh.p, cn.p);
-                break;
-            case PREC_FP16:
-                k_pack<PREC_FP16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs.p, xh.p, cn.p);
-                break;
-            default: throw HipError("[ncvis_amd] knn: unknown precision");
+    k_zero_int<<<1, 1, 0, st>>>(err.p);
+    KnnEvents ev{};
+    NCVIS_HIP_CHECK(hipEventCreate(&ev.tiles_begin));
+    NCVIS_HIP_CHECK(hipEventCreate(&ev.tiles_end));
+    NCVIS_HIP_CHECK(hipEventCreate(&ev.rerank_end));
+    struct EvGuard {
+        KnnEvents &e;
+        ~EvGuard() {
+            (void)hipEventDestroy(e.tiles_begin);
+            (void)hipEventDestroy(e.tiles_end);
+            (void)hipEventDestroy(e.rerank_end);
         }
-        NCVIS_LAUNCH_CHECK();
-        const int C = lists * KL;  // candidates per query for the rerank: 32, 64 or 128
-        DevBuf<int32_t> cand((size_t)N * C);
-        const float beta = is_l2 ? -2.0f : -1.0f;
-        NCVIS_HIP_CHECK(hipEventRecord(e0, st));
-        if (wide32)
-            knn_tiles_wide_f32(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
-        else if (wide)
-            knn_tiles_wide(prec, KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
-        else if (KL == 16)
-            knn_tiles16(prec, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
-        else if (prec == PREC_FP32 && tuning().knn_dma != 0)
-            knn_tiles_q1_f32dma(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
-        else if (prec == PREC_FP32)
-            knn_tiles_q1_f32(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
-        else if (prec == PREC_BF16)
-            knn_tiles_q1_bf16(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
-        else
-            knn_tiles_q1_fp16(KL, DK, N, Npad, xh.p, cn.p, beta, cand.p, st);
-        NCVIS_LAUNCH_CHECK();
-        NCVIS_HIP_CHECK(hipEventRecord(e1, st));
-        const int vec4 = (D % 4 == 0) && ((uintptr_t)Xp % 16 == 0);
-        if (C == 32)
-            k_knn_rerank<32><<<cdiv(N, 8), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand.p, k, d_inds, d_dists, err.p);
-        else if (C == 64)
-            k_knn_rerank<64><<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand.p, k, d_inds, d_dists, err.p);
-        else
-            k_knn_rerank128<<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand.p, k, d_inds, d_dists, err.p);
-        NCVIS_LAUNCH_CHECK();
-        NCVIS_HIP_CHECK(hipEventRecord(e2, st));
-    } else {
-        DevBuf<float> hd((size_t)N * (k + 1));
-        DevBuf<int> hi((size_t)N * (k + 1));
-        NCVIS_HIP_CHECK(hipEventRecord(e0, st));
-        k_knn_generic<<<cdiv(N, 256), 256, 0, st>>>(Xp, N, D, is_l2, k, hd.p, hi.p, d_inds, d_dists);
-        NCVIS_LAUNCH_CHECK();
-        NCVIS_HIP_CHECK(hipEventRecord(e1, st));
-        NCVIS_HIP_CHECK(hipEventRecord(e2, st));
-    }
+    } evg{ev};
+    knn_enqueue(dX, N, D, k, dist, prec, d_inds, d_dists, ws.p, err.p, st, &ev);
     int herr = 0;
     NCVIS_HIP_CHECK(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
     NCVIS_HIP_CHECK(hipStreamSynchronize(st));
     if (stats) {
         float t1 = 0, t2 = 0;
-        NCVIS_HIP_CHECK(hipEventElapsedTime(&t1, e0, e1));
-        NCVIS_HIP_CHECK(hipEventElapsedTime(&t2, e1, e2));
+        NCVIS_HIP_CHECK(hipEventElapsedTime(&t1, ev.tiles_begin, ev.tiles_end));
+        NCVIS_HIP_CHECK(hipEventElapsedTime(&t2, ev.tiles_end, ev.rerank_end));
         stats->tile_ms = t1;
         stats->rerank_ms = t2;
-        stats->used_mfma = tiled ? 1 : 0;
+        stats->used_mfma = knn_layout(N, D, k, dist, prec).tiled ? 1 : 0;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
     if (herr) throw HipError("[ncvis_amd] knn: fewer than k+1 finite candidates (non-finite input?)");
 }
 

@@ -1,10 +1,13 @@
 // Command-line harness with the reference's arguments and synthetic data (src/main.cpp:9-44):
 //   ncvis_cli [number of points] [number of neighbors] [number of threads] [maximum epochs]
 //             [number of init epochs] [knn precision: fp16 | bf16 | fp32 (optional)]
+//             [output file for Y as raw float32 (optional)]
 // X is N x 784 float32 drawn as uniform_real<float>(0, 1) from pcg64(42), the embedding has
 // d = 2, and the class is built as the reference does (M = 8, ef_construction = 200, seed 42,
 // a = b = 1, default noise plan). Stage times of the fit are printed in the reference's
 // "<stage>: <ms> ms" form (src/ncvis.cpp:370-439), then the whole call, then Y when N <= 5.
+// The number of threads sets the SGD chain count only with NCVIS_EXACT_THREADS=1 in the
+// environment (then 1 reproduces the reference at one thread); otherwise the GPU picks it.
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -43,7 +46,7 @@ int main(int argc, char **argv) {
     if (argc < 6) {
         std::printf(
             "Usage: ncvis_cli [number of points] [number of neighbors] [number of threads] [maximum epochs] "
-            "[number of init epochs] [knn precision: fp16|bf16|fp32]\n");
+            "[number of init epochs] [knn precision: fp16|bf16|fp32] [output file]\n");
         return 1;
     }
     const long N = std::atol(argv[1]), D = 28 * 28, d = 2;
@@ -69,6 +72,8 @@ int main(int argc, char **argv) {
             }
             vis.set_option("knn_precision", code);
         }
+        const char *exact = std::getenv("NCVIS_EXACT_THREADS");
+        if (exact && std::strcmp(exact, "0") != 0) vis.set_option("exact_threads", 1);
         const auto t0 = std::chrono::steady_clock::now();
         vis.fit_transform(X.data(), (size_t)N, (size_t)D, Y.data());
         const auto t1 = std::chrono::steady_clock::now();
@@ -85,6 +90,15 @@ int main(int argc, char **argv) {
     } catch (const std::exception &e) {
         std::fprintf(stderr, "%s\n", e.what());
         return 2;
+    }
+    if (argc > 7) {
+        FILE *f = std::fopen(argv[7], "wb");
+        if (!f || std::fwrite(Y.data(), sizeof(float), Y.size(), f) != Y.size()) {
+            std::fprintf(stderr, "cannot write %s\n", argv[7]);
+            if (f) std::fclose(f);
+            return 1;
+        }
+        std::fclose(f);
     }
     if (N <= 5) {
         std::printf("-----------------\n");

@@ -17,6 +17,8 @@
 // (bits/stl_algo.h:1796-1960, bits/stl_heap.h:130-430) step for step. Disjoint sub-ranges are
 // independent, so they are processed from an explicit stack (larger part deferred) instead of
 // by recursion; that changes the visiting order only, not the result.
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 
 #include "common.h"
@@ -207,23 +209,30 @@ __device__ void dev_sort_ids(int64_t *v, int64_t n) {
 }
 
 // ------------------------------------------------------------------ kernels
+template <typename T>
+__global__ void k_fill(T *__restrict__ p, int64_t n, T v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void k_count_in(const int32_t *__restrict__ inds, int64_t Nk, int64_t N, unsigned long long *__restrict__ cnt,
                            int *__restrict__ err) {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= Nk) return;
     int32_t t = inds[e];
     if (t < 0 || t >= N) {
-        atomicExch(err, 1);
+        atomicOr(err, kErrSymRange);
         return;
     }
     atomicAdd(&cnt[t], 1ull);
 }
 
-__global__ void k_scatter_in(const int32_t *__restrict__ inds, int64_t Nk, const int64_t *__restrict__ in_ptr,
+__global__ void k_scatter_in(const int32_t *__restrict__ inds, int64_t Nk, int64_t N, const int64_t *__restrict__ in_ptr,
                              unsigned long long *__restrict__ fill, int64_t *__restrict__ in_src) {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= Nk) return;
     int32_t t = inds[e];
+    if (t < 0 || t >= N) return;  // flagged by k_count_in
     int64_t pos = in_ptr[t] + (int64_t)atomicAdd(&fill[t], 1ull);
     in_src[pos] = e;
 }
@@ -283,55 +292,99 @@ __global__ void k_compact(int64_t N, int k, const int64_t *__restrict__ in_ptr, 
     }
 }
 
-// exclusive prefix sum of `n` values into out[0..n], out[n] = total
+// exclusive prefix sum of `n` values into out[0..n], out[n] = total (no host synchronisation)
 template <typename In>
-static void exclusive_scan(const In *d_in, int64_t *d_out, int64_t n, hipStream_t st) {
-    // out[0] = 0, out[1..n] = inclusive scan of in[0..n)
-    NCVIS_HIP_CHECK(hipMemsetAsync(d_out, 0, sizeof(int64_t), st));
+static void exclusive_scan(const In *d_in, int64_t *d_out, int64_t n, void *tmp, size_t tmp_bytes, hipStream_t st) {
+    k_fill<int64_t><<<1, 1, 0, st>>>(d_out, 1, 0);
+    NCVIS_LAUNCH_CHECK();
     if (n == 0) return;
-    size_t tmp_bytes = 0;
-    NCVIS_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, d_in, d_out + 1, (int)n, st));
-    DevBuf<unsigned char> tmp(tmp_bytes);
-    NCVIS_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(tmp.p, tmp_bytes, d_in, d_out + 1, (int)n, st));
-    NCVIS_HIP_CHECK(hipStreamSynchronize(st));
+    size_t bytes = tmp_bytes;
+    NCVIS_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(tmp, bytes, d_in, d_out + 1, (int)n, st));
+}
+
+// Workspace of symmetrize_enqueue, carved at 256-byte boundaries.
+struct SymLayout {
+    size_t cnt, in_ptr, in_src, tind, idx, alias, tdist, outd, rowlen, tmp, tmp_bytes, total;
+};
+
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+static SymLayout sym_layout(int64_t N, int k) {
+    const int64_t Nk = N * (int64_t)k;
+    const size_t cap = (size_t)(2 * Nk);
+    size_t a = 0, b = 0;
+    NCVIS_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, a, (const unsigned long long *)nullptr,
+                                                     (int64_t *)nullptr, (int)N));
+    NCVIS_HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, b, (const int64_t *)nullptr, (int64_t *)nullptr,
+                                                     (int)N));
+    SymLayout L;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align256(o + bytes);
+        return at;
+    };
+    L.cnt = take(sizeof(unsigned long long) * N);
+    L.in_ptr = take(sizeof(int64_t) * (N + 1));
+    L.in_src = take(sizeof(int64_t) * Nk);
+    L.tind = take(sizeof(int32_t) * cap);
+    L.idx = take(sizeof(int32_t) * cap);
+    L.alias = take(sizeof(int32_t) * cap);
+    L.tdist = take(sizeof(float) * cap);
+    L.outd = take(sizeof(float) * cap);
+    L.rowlen = take(sizeof(int64_t) * N);
+    L.tmp_bytes = std::max<size_t>(std::max(a, b), 1);
+    L.tmp = take(L.tmp_bytes);
+    L.total = o;
+    return L;
+}
+
+size_t symmetrize_workspace_bytes(int64_t N, int k) { return sym_layout(N, k).total; }
+
+void symmetrize_enqueue(const int32_t *d_inds, const float *d_dists, int64_t N, int k, int64_t *d_row_ptr,
+                        int32_t *d_col, float *d_dist, void *ws, int *d_err, hipStream_t st) {
+    if (N < 1 || k < 1) throw HipError("[ncvis_amd] symmetrize: invalid sizes");
+    if (N > INT32_MAX) throw HipError("[ncvis_amd] symmetrize: more than 2^31 points");
+    const SymLayout L = sym_layout(N, k);
+    unsigned char *w = (unsigned char *)ws;
+    auto *cnt = (unsigned long long *)(w + L.cnt);
+    auto *in_ptr = (int64_t *)(w + L.in_ptr);
+    auto *in_src = (int64_t *)(w + L.in_src);
+    auto *tind = (int32_t *)(w + L.tind);
+    auto *idx = (int32_t *)(w + L.idx);
+    auto *alias = (int32_t *)(w + L.alias);
+    auto *tdist = (float *)(w + L.tdist);
+    auto *outd = (float *)(w + L.outd);
+    auto *rowlen = (int64_t *)(w + L.rowlen);
+    void *tmp = w + L.tmp;
+    const int64_t Nk = N * (int64_t)k;
+    k_fill<unsigned long long><<<cdiv(N, 256), 256, 0, st>>>(cnt, N, 0ull);
+    k_count_in<<<cdiv(Nk, 256), 256, 0, st>>>(d_inds, Nk, N, cnt, d_err);
+    NCVIS_LAUNCH_CHECK();
+    exclusive_scan(cnt, in_ptr, N, tmp, L.tmp_bytes, st);
+    k_fill<unsigned long long><<<cdiv(N, 256), 256, 0, st>>>(cnt, N, 0ull);  // reused as fill counters
+    k_scatter_in<<<cdiv(Nk, 256), 256, 0, st>>>(d_inds, Nk, N, in_ptr, cnt, in_src);
+    NCVIS_LAUNCH_CHECK();
+    k_merge_rows<<<cdiv(N, 128), 128, 0, st>>>(d_inds, d_dists, N, k, in_ptr, in_src, tind, tdist, idx, alias, outd,
+                                                rowlen);
+    NCVIS_LAUNCH_CHECK();
+    exclusive_scan(rowlen, d_row_ptr, N, tmp, L.tmp_bytes, st);
+    k_compact<<<cdiv(N, 256), 256, 0, st>>>(N, k, in_ptr, d_row_ptr, alias, outd, d_col, d_dist);
+    NCVIS_LAUNCH_CHECK();
 }
 
 int64_t symmetrize_run(const int32_t *d_inds, const float *d_dists, int64_t N, int k, int64_t *d_row_ptr,
                        int32_t *d_col, float *d_dist, hipStream_t st) {
-    if (N < 1 || k < 1) throw HipError("[ncvis_amd] symmetrize: invalid sizes");
-    if (N > INT32_MAX) throw HipError("[ncvis_amd] symmetrize: more than 2^31 points");
-    const int64_t Nk = N * (int64_t)k;
-    DevBuf<unsigned long long> cnt((size_t)N);
+    DevBuf<unsigned char> ws(symmetrize_workspace_bytes(N, k));
     DevBuf<int> err(1);
-    NCVIS_HIP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * N, st));
-    NCVIS_HIP_CHECK(hipMemsetAsync(err.p, 0, sizeof(int), st));
-    k_count_in<<<cdiv(Nk, 256), 256, 0, st>>>(d_inds, Nk, N, cnt.p, err.p);
-    NCVIS_LAUNCH_CHECK();
+    k_fill<int><<<1, 1, 0, st>>>(err.p, 1, 0);
+    symmetrize_enqueue(d_inds, d_dists, N, k, d_row_ptr, d_col, d_dist, ws.p, err.p, st);
     int herr = 0;
-    NCVIS_HIP_CHECK(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    NCVIS_HIP_CHECK(hipStreamSynchronize(st));
-    if (herr) throw HipError("[ncvis_amd] symmetrize: neighbour index out of range");
-
-    DevBuf<int64_t> in_ptr((size_t)N + 1);
-    exclusive_scan(cnt.p, in_ptr.p, N, st);
-    NCVIS_HIP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * N, st));  // reuse as fill
-    DevBuf<int64_t> in_src((size_t)Nk);
-    k_scatter_in<<<cdiv(Nk, 256), 256, 0, st>>>(d_inds, Nk, in_ptr.p, cnt.p, in_src.p);
-    NCVIS_LAUNCH_CHECK();
-
-    const size_t cap = (size_t)(2 * Nk);
-    DevBuf<int32_t> tind(cap), idx(cap), alias(cap);
-    DevBuf<float> tdist(cap), outd(cap);
-    DevBuf<int64_t> rowlen((size_t)N);
-    k_merge_rows<<<cdiv(N, 128), 128, 0, st>>>(d_inds, d_dists, N, k, in_ptr.p, in_src.p, tind.p, tdist.p, idx.p,
-                                                alias.p, outd.p, rowlen.p);
-    NCVIS_LAUNCH_CHECK();
-    exclusive_scan(rowlen.p, d_row_ptr, N, st);
-    k_compact<<<cdiv(N, 256), 256, 0, st>>>(N, k, in_ptr.p, d_row_ptr, alias.p, outd.p, d_col, d_dist);
-    NCVIS_LAUNCH_CHECK();
     int64_t E = 0;
+    NCVIS_HIP_CHECK(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
     NCVIS_HIP_CHECK(hipMemcpyAsync(&E, d_row_ptr + N, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     NCVIS_HIP_CHECK(hipStreamSynchronize(st));
+    if (herr & kErrSymRange) throw HipError("[ncvis_amd] symmetrize: neighbour index out of range");
     return E;
 }
 
