@@ -120,6 +120,23 @@ int ncvis_optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const in
                           float alpha_Q, uint64_t seed, int64_t n_threads, int atomic_updates, float *q_trace,
                           void *stream);
 
+/* ---- kNN tile autotuning (no reference counterpart: the reference's HNSW has no tiles) ----
+ * ncvis_autotune_knn times every admissible launch variant of the kNN candidate tiles for one
+ * search shape (candidate split 1/2/4, the 16-bit LDS stage size, FP32 LDS DMA) on dX [N, D]
+ * (device rows; NULL: uniform synthetic rows) and keeps the fastest in a process-wide table keyed
+ * by device architecture, GEMM precision, padded width, list length and floor(log2 N). Every
+ * variant gives the exact table on exactly scored data; more splits only add candidates to the
+ * exact rerank. cache_path: NULL keeps the table in memory, "" writes it to
+ * $NCVIS_AMD_TUNE_CACHE, any other string writes it to that JSON file. Outputs (may be NULL): the
+ * chosen variant and its kNN time in ms. The table is read from $NCVIS_AMD_TUNE_CACHE at the first
+ * search of a process; ncvis_load_tune_cache replaces it from a file (NULL or "": empties it).
+ * Errors: ncvis_autotune_last_error. */
+int ncvis_autotune_knn(const float *dX, size_t N, size_t D, size_t k, int dist, int precision, const char *cache_path,
+                       int *split_out, int *stage_mult_out, int *dma_out, double *ms_out);
+int ncvis_load_tune_cache(const char *path);
+int ncvis_tune_cache_size(void);
+const char *ncvis_autotune_last_error(void);
+
 /* ncvis_optimize_device without any device allocation. No counterpart in the reference: it is
  * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc). Same arguments and results as ncvis_optimize_device;

@@ -17,6 +17,7 @@ EXPORTS = (
     "ncvis_fit_transform_device", "ncvis_get_stats", "ncvis_get_last_error", "ncvis_knn_device",
     "ncvis_symmetrize_device", "ncvis_init_embedding_device", "ncvis_optimize_device",
     "ncvis_nce_loss_device", "ncvis_tune", "ncvis_fit_transform_many", "ncvis_optimize_device_noalloc",
+    "ncvis_autotune_knn", "ncvis_load_tune_cache", "ncvis_tune_cache_size", "ncvis_autotune_last_error",
 )
 
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
@@ -61,11 +62,18 @@ def lib():
                                             C.c_int64, C.c_int, vp, vp]
         L.ncvis_optimize_device_noalloc.argtypes = L.ncvis_optimize_device.argtypes
         L.ncvis_tune.argtypes = [C.c_char_p, C.c_double]
+        L.ncvis_autotune_knn.argtypes = [vp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int, C.c_int, C.c_char_p,
+                                         C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_double)]
+        L.ncvis_load_tune_cache.argtypes = [C.c_char_p]
+        L.ncvis_tune_cache_size.argtypes = []
+        L.ncvis_autotune_last_error.argtypes = []
+        L.ncvis_autotune_last_error.restype = C.c_char_p
         L.ncvis_nce_loss_device.argtypes = [C.c_size_t, C.c_size_t, vp, vp, C.c_int64, vp, C.c_int,
                                             vp, C.c_float, C.c_float, C.c_float, vp, vp, vp, vp]
         for name in EXPORTS:
             f = getattr(L, name)
-            if name not in ("ncvis_destroy", "ncvis_last_error", "ncvis_get_last_error"):
+            if name not in ("ncvis_destroy", "ncvis_last_error", "ncvis_get_last_error", "ncvis_autotune_last_error"):
                 f.restype = C.c_int
         _lib = L
     return _lib
@@ -78,6 +86,27 @@ def _check(rc: int) -> None:
 
 def _stream(stream) -> int:
     return 0 if stream is None else int(stream)
+
+
+def autotune_knn(dX, N: int, D: int, k: int, distance: str = "euclidean", precision: str = "fp16",
+                 cache_path=None):
+    """Times the kNN tile variants for this search shape (dX: device pointer, or None for synthetic
+    rows) and keeps the fastest; cache_path: None (memory only), "" ($NCVIS_AMD_TUNE_CACHE) or a
+    JSON file. Returns {"split", "stage_mult", "dma", "ms"}."""
+    sp, sm, dm, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    path = None if cache_path is None else str(cache_path).encode()
+    rc = lib().ncvis_autotune_knn(dX or None, N, D, k, DIST[distance], PREC[precision], path, C.byref(sp),
+                                  C.byref(sm), C.byref(dm), C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(lib().ncvis_autotune_last_error().decode())
+    return {"split": sp.value, "stage_mult": sm.value, "dma": dm.value, "ms": ms.value}
+
+
+def load_tune_cache(path=None) -> int:
+    """Replaces the tuning table from a JSON file (None: empties it); returns its entry count."""
+    if lib().ncvis_load_tune_cache(None if path is None else str(path).encode()) != 0:
+        raise RuntimeError(lib().ncvis_autotune_last_error().decode())
+    return lib().ncvis_tune_cache_size()
 
 
 def tune(key: str, value: float) -> None:

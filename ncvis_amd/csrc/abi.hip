@@ -2,6 +2,7 @@
 // and the device-pointer stage entry points. Declarations and reference citations are in
 // include/ncvis_amd.h.
 #include <algorithm>
+#include <numeric>
 #include <atomic>
 #include <memory>
 #include <cstring>
@@ -137,22 +138,59 @@ struct FitEvents {
     }
 };
 
-// A fit of one or many data sets with one parameter set: kNN and symmetrize per data set (their
-// scratch shared, in stream order), then the segmented chain counts, init and SGD epochs of all
-// data sets at once, every stage queued without host synchronisation. Its memory is one arena
-// that stays allocated for the next fit of the same shapes; a batch can be captured in a
-// hipGraph and replayed while its inputs stay the same.
+// CSR of a symmetrize group (several data sets' rows concatenated, global indices) split into
+// each data set's own row_ptr / col with local indices. One thread per group row.
+__global__ void k_split_csr(const FitSeg *__restrict__ s, int B, int64_t base_row, int64_t rows,
+                            const int64_t *__restrict__ rp_g, const int32_t *__restrict__ col_g) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int64_t gr = base_row + r;
+    int lo = 0, hi = B;  // data set holding global row gr
+    while (hi - lo > 1) {
+        const int m = (lo + hi) >> 1;
+        if (s[m].row_off <= gr)
+            lo = m;
+        else
+            hi = m;
+    }
+    const FitSeg &g = s[lo];
+    const int64_t ro = g.row_off - base_row;  // first group row of the data set
+    const int64_t i = r - ro;
+    const int64_t eb = rp_g[ro], e0 = rp_g[r], e1 = rp_g[r + 1];
+    int64_t *row_ptr = const_cast<int64_t *>(g.row_ptr);
+    int32_t *col = const_cast<int32_t *>(g.col);
+    row_ptr[i] = e0 - eb;
+    if (i == g.N - 1) row_ptr[g.N] = e1 - eb;
+    for (int64_t e = e0; e < e1; ++e) col[e - eb] = col_g[e] - (int32_t)ro;
+}
+
+// A fit of one or many data sets with one parameter set, every stage queued without host
+// synchronisation. Data sets are ordered by their k (n_neighbors clamped to N - 1); each run of
+// equal k is one symmetrize group: the kNN tables of the group's data sets are written into one
+// table with global row indices and symmetrized in one pass (a row references only rows of its
+// own data set, and incoming lists stay in ascending source order, so the result equals each data
+// set's own symmetrize), then split into per data set CSR. The chain counts, init and SGD epochs
+// of all data sets run segmented, one launch each. The memory is one arena that stays allocated
+// for the next fit of the same shapes; a batch can be captured in a hipGraph and replayed while
+// its inputs stay the same.
 struct FitBatch {
-    std::vector<FitJob> jobs;
-    uint64_t gen = 0;  // handle and tuning generation the plan was made for
-    std::vector<int> ks;
-    std::vector<FitSeg> segs;
+    std::vector<FitJob> jobs;   // in the caller's order
+    std::vector<int> order;     // descriptor position -> caller's index (sorted by k, stable)
+    uint64_t gen = 0;           // handle and tuning generation the plan was made for
+    std::vector<int> ks;        // per descriptor position
+    std::vector<FitSeg> segs;   // per descriptor position
+    struct Group {
+        int b0, nb, k;
+        int64_t row0, rows;
+    };
+    std::vector<Group> groups;
     SegTotals tot;
     DevBuf<unsigned char> arena;
     unsigned char *knn_ws = nullptr, *sym_ws = nullptr;
-    int32_t *inds = nullptr;
+    int32_t *inds = nullptr, *col_g = nullptr;
     float *dists = nullptr, *cdist = nullptr;
-    int *errs = nullptr;
+    int64_t *rp_g = nullptr;
+    int *errs = nullptr;  // [B] kNN flags per descriptor position, then [groups] symmetrize flags
     FitSeg *d_segs = nullptr;
     hipGraphExec_t exec = nullptr;
 
@@ -176,57 +214,75 @@ struct FitBatch {
         const int B = (int)jobs.size();
         const int d = (int)h->d;
         const ChainRule rule = rule_of(h);
+        std::vector<int> kk(B);
+        for (int i = 0; i < B; ++i) {
+            const size_t k0 = (h->n_neighbors < jobs[i].N - 1) ? h->n_neighbors : (jobs[i].N - 1);  // ncvis.cpp:381-382
+            kk[i] = (int)(k0 > 0 ? k0 : 1);
+        }
+        order.resize(B);
+        for (int i = 0; i < B; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return kk[x] < kk[y]; });
         ks.assign(B, 0);
         segs.assign(B, FitSeg{});
+        groups.clear();
         tot = SegTotals{};
-        auto a256 = [](size_t x) { return (x + 255) / 256 * 256; };
-        size_t knn_bytes = 0, sym_bytes = 0, nk_max = 0;
-        for (int b = 0; b < B; ++b) {
-            const FitJob &j = jobs[b];
-            size_t kk = (h->n_neighbors < j.N - 1) ? h->n_neighbors : (j.N - 1);  // ncvis.cpp:381-382
-            ks[b] = (int)(kk > 0 ? kk : 1);
-            knn_bytes = std::max(knn_bytes, knn_workspace_bytes((int64_t)j.N, (int)j.D, ks[b], h->dist, h->precision));
-            sym_bytes = std::max(sym_bytes, symmetrize_workspace_bytes((int64_t)j.N, ks[b]));
-            nk_max = std::max(nk_max, j.N * (size_t)ks[b]);
+        size_t knn_bytes = 0;
+        for (int p = 0; p < B; ++p) {
+            const FitJob &j = jobs[order[p]];
+            ks[p] = kk[order[p]];
+            knn_bytes = std::max(knn_bytes, knn_workspace_bytes((int64_t)j.N, (int)j.D, ks[p], h->dist, h->precision));
+            if (groups.empty() || groups.back().k != ks[p]) groups.push_back(Group{p, 0, ks[p], tot.rows, 0});
+            groups.back().nb += 1;
+            groups.back().rows += (int64_t)j.N;
+            segs[p].row_off = tot.rows;
+            tot.rows += (int64_t)j.N;
         }
-        // layout: shared scratch, then per data set arrays, then errors and descriptors
+        size_t sym_bytes = 0, gnk = 0, grows = 0;
+        for (const Group &gr : groups) {
+            sym_bytes = std::max(sym_bytes, symmetrize_workspace_bytes(gr.rows, gr.k));
+            gnk = std::max(gnk, (size_t)gr.rows * gr.k);
+            if (gr.nb > 1) grows = std::max(grows, (size_t)gr.rows);
+        }
+        auto a256 = [](size_t x) { return (x + 255) / 256 * 256; };
         size_t o = 0;
         auto take = [&](size_t bytes) {
             const size_t at = o;
             o = a256(o + bytes);
             return at;
         };
-        const size_t o_knn = take(knn_bytes), o_sym = take(sym_bytes), o_inds = take(sizeof(int32_t) * nk_max),
-                     o_dists = take(sizeof(float) * nk_max), o_cdist = take(sizeof(float) * 2 * nk_max);
+        // shared scratch: kNN workspace, the concatenated kNN table of a group, symmetrize
+        // workspace and output of a multi-set group
+        const size_t o_knn = take(knn_bytes), o_sym = take(sym_bytes), o_inds = take(sizeof(int32_t) * gnk),
+                     o_dists = take(sizeof(float) * gnk), o_cdist = take(sizeof(float) * 2 * gnk),
+                     o_rpg = take(grows ? sizeof(int64_t) * (grows + 1) : 0),
+                     o_colg = take(grows ? sizeof(int32_t) * 2 * gnk : 0);
         struct Off {
             size_t row_ptr, col, y2, part, stats, chain_row, rng, qpart, qdev;
         };
         std::vector<Off> off(B);
-        for (int b = 0; b < B; ++b) {
-            const FitJob &j = jobs[b];
-            FitSeg &v = segs[b];
+        for (int p = 0; p < B; ++p) {
+            const FitJob &j = jobs[order[p]];
+            FitSeg &v = segs[p];
             const int64_t N = (int64_t)j.N;
             v.N = N;
             v.nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, cdiv(N, 256)));
-            v.T_cap = chain_rule(2 * N * ks[b], rule);  // the rule never decreases with E <= 2 N k
-            v.alpha_init = (float)(1. / ks[b]);         // ncvis.cpp:418
+            v.T_cap = chain_rule(2 * N * ks[p], rule);  // the rule never decreases with E <= 2 N k
+            v.alpha_init = (float)(1. / ks[p]);         // ncvis.cpp:418
             v.pairs_ok = ((uintptr_t)j.dY % 8 == 0) ? 1 : 0;
-            v.row_off = tot.rows;
             v.chain_off = tot.slots;
-            tot.rows += N;
             tot.slots += v.T_cap;
             tot.max_nb = std::max(tot.max_nb, v.nb);
-            off[b].row_ptr = take(sizeof(int64_t) * (N + 1));
-            off[b].col = take(sizeof(int32_t) * 2 * N * ks[b]);
-            off[b].y2 = take(h->n_init_epochs > 0 ? sizeof(float) * N * d : 0);
-            off[b].part = take(sizeof(double) * v.nb * d);
-            off[b].stats = take(sizeof(float) * 2 * d);
-            off[b].chain_row = take(sizeof(int64_t) * v.T_cap);
-            off[b].rng = take(sizeof(U128) * v.T_cap);
-            off[b].qpart = take(sizeof(float) * v.T_cap);
-            off[b].qdev = take(sizeof(float) * (1 + (size_t)std::max(h->n_epochs, 0)));
+            off[p].row_ptr = take(sizeof(int64_t) * (N + 1));
+            off[p].col = take(sizeof(int32_t) * 2 * N * ks[p]);
+            off[p].y2 = take(h->n_init_epochs > 0 ? sizeof(float) * N * d : 0);
+            off[p].part = take(sizeof(double) * v.nb * d);
+            off[p].stats = take(sizeof(float) * 2 * d);
+            off[p].chain_row = take(sizeof(int64_t) * v.T_cap);
+            off[p].rng = take(sizeof(U128) * v.T_cap);
+            off[p].qpart = take(sizeof(float) * v.T_cap);
+            off[p].qdev = take(sizeof(float) * (1 + (size_t)std::max(h->n_epochs, 0)));
         }
-        const size_t o_errs = take(sizeof(int) * B), o_segs = take(sizeof(FitSeg) * B);
+        const size_t o_errs = take(sizeof(int) * (B + groups.size())), o_segs = take(sizeof(FitSeg) * B);
         arena.alloc(o);
         unsigned char *w = arena.p;
         knn_ws = w + o_knn;
@@ -234,21 +290,29 @@ struct FitBatch {
         inds = (int32_t *)(w + o_inds);
         dists = (float *)(w + o_dists);
         cdist = (float *)(w + o_cdist);
+        rp_g = grows ? (int64_t *)(w + o_rpg) : nullptr;
+        col_g = grows ? (int32_t *)(w + o_colg) : nullptr;
         errs = (int *)(w + o_errs);
         d_segs = (FitSeg *)(w + o_segs);
-        for (int b = 0; b < B; ++b) {
-            FitSeg &v = segs[b];
-            v.Y = jobs[b].dY;
-            v.Y2 = h->n_init_epochs > 0 ? (float *)(w + off[b].y2) : nullptr;
-            v.row_ptr = (const int64_t *)(w + off[b].row_ptr);
-            v.col = (const int32_t *)(w + off[b].col);
-            v.part = (double *)(w + off[b].part);
-            v.stats = (float *)(w + off[b].stats);
-            v.chain_row = (int64_t *)(w + off[b].chain_row);
-            v.rng = (U128 *)(w + off[b].rng);
-            v.qpart = (float *)(w + off[b].qpart);
-            v.qdev = (float *)(w + off[b].qdev);
+        for (int p = 0; p < B; ++p) {
+            FitSeg &v = segs[p];
+            v.Y = jobs[order[p]].dY;
+            v.Y2 = h->n_init_epochs > 0 ? (float *)(w + off[p].y2) : nullptr;
+            v.row_ptr = (const int64_t *)(w + off[p].row_ptr);
+            v.col = (const int32_t *)(w + off[p].col);
+            v.part = (double *)(w + off[p].part);
+            v.stats = (float *)(w + off[p].stats);
+            v.chain_row = (int64_t *)(w + off[p].chain_row);
+            v.rng = (U128 *)(w + off[p].rng);
+            v.qpart = (float *)(w + off[p].qpart);
+            v.qdev = (float *)(w + off[p].qdev);
         }
+    }
+
+    // new input and output pointers for the same shapes (single fits reuse their arena)
+    void repoint(const std::vector<FitJob> &js) {
+        jobs = js;
+        for (size_t p = 0; p < segs.size(); ++p) segs[p].Y = jobs[order[p]].dY;
     }
 
     void upload(hipStream_t st) {
@@ -258,17 +322,29 @@ struct FitBatch {
     // every stage of every data set on `st`; ev (single eager fits only) brackets the stages
     void enqueue(const ncvis_handle *h, hipStream_t st, FitEvents *ev) {
         const int B = (int)jobs.size();
+        const int G = (int)groups.size();
         const int d = (int)h->d;
-        k_zero_ints<<<cdiv(B, 256), 256, 0, st>>>(errs, B);
+        k_zero_ints<<<cdiv(B + G, 256), 256, 0, st>>>(errs, B + G);
         NCVIS_LAUNCH_CHECK();
         if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[0], st));
-        for (int b = 0; b < B; ++b) {
-            const FitJob &j = jobs[b];
-            knn_enqueue(j.dX, (int64_t)j.N, (int)j.D, ks[b], h->dist, h->precision, inds, dists, knn_ws, errs + b, st,
-                        ev ? &ev->knn : nullptr);
+        for (int gi = 0; gi < G; ++gi) {
+            const Group &gr = groups[gi];
+            for (int p = gr.b0; p < gr.b0 + gr.nb; ++p) {
+                const FitJob &j = jobs[order[p]];
+                const int64_t local = segs[p].row_off - gr.row0;
+                knn_enqueue(j.dX, (int64_t)j.N, (int)j.D, gr.k, h->dist, h->precision, inds + local * gr.k,
+                            dists + local * gr.k, knn_ws, errs + p, st, ev ? &ev->knn : nullptr, nullptr,
+                            (int32_t)local);
+            }
             if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[1], st));
-            symmetrize_enqueue(inds, dists, (int64_t)j.N, ks[b], const_cast<int64_t *>(segs[b].row_ptr),
-                               const_cast<int32_t *>(segs[b].col), cdist, sym_ws, errs + b, st);
+            if (gr.nb == 1) {
+                symmetrize_enqueue(inds, dists, gr.rows, gr.k, const_cast<int64_t *>(segs[gr.b0].row_ptr),
+                                   const_cast<int32_t *>(segs[gr.b0].col), cdist, sym_ws, errs + B + gi, st);
+            } else {
+                symmetrize_enqueue(inds, dists, gr.rows, gr.k, rp_g, col_g, cdist, sym_ws, errs + B + gi, st);
+                k_split_csr<<<cdiv(gr.rows, 256), 256, 0, st>>>(d_segs + gr.b0, gr.nb, gr.row0, gr.rows, rp_g, col_g);
+                NCVIS_LAUNCH_CHECK();
+            }
         }
         if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[2], st));
         seg_dims(d_segs, B, rule_of(h), d, st);
@@ -309,22 +385,24 @@ struct FitBatch {
     // error flags of every data set (after the stream has finished); throws for the first
     void check_errors(hipStream_t st, bool name_set) {
         const int B = (int)jobs.size();
-        std::vector<int> herr(B, 0);
-        NCVIS_HIP_CHECK(hipMemcpyAsync(herr.data(), errs, sizeof(int) * B, hipMemcpyDeviceToHost, st));
+        const int G = (int)groups.size();
+        std::vector<int> herr(B + G, 0);
+        NCVIS_HIP_CHECK(hipMemcpyAsync(herr.data(), errs, sizeof(int) * (B + G), hipMemcpyDeviceToHost, st));
         NCVIS_HIP_CHECK(hipStreamSynchronize(st));
-        for (int b = 0; b < B; ++b) {
-            if (!herr[b]) continue;
-            std::string m = (herr[b] & kErrKnnCandidates)
-                                ? "[ncvis_amd] knn: fewer than k+1 finite candidates (non-finite input?)"
-                                : "[ncvis_amd] symmetrize: neighbour index out of range";
-            if (name_set) m = "[ncvis_amd] fit_transform_many: data set " + std::to_string(b) + ": " + m;
+        int first = -1;
+        for (int p = 0; p < B; ++p)
+            if (herr[p] && (first < 0 || order[p] < order[first])) first = p;
+        if (first >= 0) {
+            std::string m = "[ncvis_amd] knn: fewer than k+1 finite candidates (non-finite input?)";
+            if (name_set) m = "[ncvis_amd] fit_transform_many: data set " + std::to_string(order[first]) + ": " + m;
             throw HipError(m);
         }
+        for (int gi = 0; gi < G; ++gi)
+            if (herr[B + gi]) throw HipError("[ncvis_amd] symmetrize: neighbour index out of range");
     }
 };
 
 // handle parameters and process-wide knobs change a plan: both count generations
-static std::atomic<uint64_t> g_tuning_gen{0};
 static uint64_t plan_gen(const ncvis_handle *h);
 
 struct FitStats {
@@ -411,7 +489,7 @@ static unsigned char *noalloc_scratch(hipStream_t st) {
     return ptrs[dev];
 }
 
-static uint64_t plan_gen(const ncvis_handle *h) { return (h->gen << 20) ^ g_tuning_gen.load(); }
+static uint64_t plan_gen(const ncvis_handle *h) { return (h->gen << 24) ^ tuning_generation(); }
 
 static void fit_device(ncvis_handle *h, const float *dX, size_t N, size_t D, float *dY, hipStream_t st,
                        FitStats &out) {
@@ -423,8 +501,7 @@ static void fit_device(ncvis_handle *h, const float *dX, size_t N, size_t D, flo
     const bool same_shape = fb.gen == g && fb.jobs.size() == 1 && fb.jobs[0].N == N && fb.jobs[0].D == D &&
                             (((uintptr_t)fb.jobs[0].dY % 8 == 0) == ((uintptr_t)dY % 8 == 0));
     if (same_shape) {
-        fb.jobs = js;  // same arena, new input and output pointers
-        fb.segs[0].Y = dY;
+        fb.repoint(js);  // same arena, new input and output pointers
     } else {
         fb.plan(h, js, g);
     }
@@ -532,7 +609,7 @@ int ncvis_tune(const char *key, double value) {
     return guarded(g_last_error, [&] {
         std::string k = key ? key : "";
         Tuning &t = tuning();
-        ++g_tuning_gen;
+        bump_tuning_generation();
         if (k == "sgd_prefetch") {
             int v = (int)value;
             if (v != 1 && v != 2 && v != 4 && v != 8) throw std::invalid_argument("[ncvis_amd] sgd_prefetch: 1, 2, 4 or 8");
@@ -546,6 +623,10 @@ int ncvis_tune(const char *key, double value) {
             t.edges_per_chain = (int64_t)value;
         } else if (k == "knn_dma") {
             t.knn_dma = value < 0 ? -1 : (value != 0);
+        } else if (k == "knn_split") {
+            if (value != 0 && value != 1 && value != 2 && value != 4)
+                throw std::invalid_argument("[ncvis_amd] knn_split: 0 (auto), 1, 2 or 4");
+            t.knn_split = (int)value;
         } else if (k == "knn_stage_mult") {
             if (value != 1 && value != 2) throw std::invalid_argument("[ncvis_amd] knn_stage_mult: 1 or 2");
             t.knn_stage_mult = (int)value;

@@ -1,0 +1,366 @@
+// Autotuner of the kNN candidate tiles and its on-disk cache.
+//
+// The launch choices a TileVariant holds (candidate split, 16-bit LDS stage multiplier, FP32 LDS
+// DMA) never change which candidates a list keeps from its range, so every variant yields the
+// exact table on exactly scored data, and more splits only add candidates to the exact rerank.
+// Their speed, though, depends on the search's shape: a small data set needs the candidate
+// split to fill 256 CUs, a large one does not. ncvis_autotune_knn times every admissible variant
+// for one shape (on the caller's rows or on uniform synthetic rows) and keeps the fastest in a
+// process-wide table keyed by (device architecture, GEMM precision, padded width, list length,
+// row-band, floor(log2 N)); the table is written to and read from a JSON file (the path given,
+// or $NCVIS_AMD_TUNE_CACHE, which is also loaded at the first search of a process). Searches
+// without an entry use the default split rule of knn.hip.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "ncvis_amd.h"
+
+namespace ncvis_hip {
+
+static std::atomic<uint64_t> g_generation{1};
+uint64_t tuning_generation() { return g_generation.load(); }
+void bump_tuning_generation() { ++g_generation; }
+
+namespace {
+
+struct TuneKey {
+    std::string arch;
+    int prec, dk, kl, wide, nlog2;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(arch, prec, dk, kl, wide, nlog2) < std::tie(o.arch, o.prec, o.dk, o.kl, o.wide, o.nlog2);
+    }
+};
+
+struct TuneEntry {
+    TileVariant tv;
+    double ms = 0.0;
+};
+
+struct TuneTable {
+    std::mutex mu;
+    std::map<TuneKey, TuneEntry> entries;
+    bool env_loaded = false;
+};
+
+TuneTable &table() {
+    static TuneTable t;
+    return t;
+}
+
+int nlog2_of(int64_t N) {
+    int l = 0;
+    while (((int64_t)1 << (l + 1)) <= N) ++l;
+    return l;
+}
+
+// gfx name of the current device without target features ("gfx950:sramecc+:xnack-" -> "gfx950")
+std::string device_arch() {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return "unknown";
+    }
+    std::string a = p.gcnArchName;
+    const size_t c = a.find(':');
+    return c == std::string::npos ? a : a.substr(0, c);
+}
+
+// ---- a small reader for the flat JSON this file writes: {"entries": [{"key": value, ...}, ...]}
+struct Obj {
+    std::map<std::string, std::string> kv;
+    int i(const char *k, int dflt) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? dflt : std::atoi(it->second.c_str());
+    }
+    double f(const char *k) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? 0.0 : std::atof(it->second.c_str());
+    }
+    std::string s(const char *k) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? std::string() : it->second;
+    }
+};
+
+std::vector<Obj> parse_entries(const std::string &txt) {
+    std::vector<Obj> out;
+    size_t p = txt.find("\"entries\"");
+    if (p == std::string::npos) throw std::invalid_argument("[ncvis_amd] tuning cache: no \"entries\" array");
+    p = txt.find('[', p);
+    if (p == std::string::npos) throw std::invalid_argument("[ncvis_amd] tuning cache: malformed \"entries\"");
+    const size_t end = txt.find(']', p);
+    while (true) {
+        const size_t a = txt.find('{', p);
+        if (a == std::string::npos || a > end) break;
+        const size_t b = txt.find('}', a);
+        if (b == std::string::npos) throw std::invalid_argument("[ncvis_amd] tuning cache: unterminated entry");
+        Obj o;
+        size_t q = a + 1;
+        while (true) {
+            const size_t k0 = txt.find('"', q);
+            if (k0 == std::string::npos || k0 > b) break;
+            const size_t k1 = txt.find('"', k0 + 1);
+            const size_t colon = txt.find(':', k1);
+            if (k1 == std::string::npos || colon == std::string::npos || colon > b)
+                throw std::invalid_argument("[ncvis_amd] tuning cache: malformed entry");
+            size_t v0 = colon + 1;
+            while (v0 < b && (txt[v0] == ' ' || txt[v0] == '\n' || txt[v0] == '\t')) ++v0;
+            size_t v1;
+            std::string val;
+            if (txt[v0] == '"') {
+                v1 = txt.find('"', v0 + 1);
+                val = txt.substr(v0 + 1, v1 - v0 - 1);
+                ++v1;
+            } else {
+                v1 = v0;
+                while (v1 < b && txt[v1] != ',' && txt[v1] != '}') ++v1;
+                val = txt.substr(v0, v1 - v0);
+            }
+            o.kv[txt.substr(k0 + 1, k1 - k0 - 1)] = val;
+            q = v1;
+        }
+        out.push_back(o);
+        p = b + 1;
+    }
+    return out;
+}
+
+void load_file_locked(TuneTable &t, const std::string &path) {
+    std::ifstream f(path);
+    if (!f) throw std::invalid_argument("[ncvis_amd] cannot read tuning cache " + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    for (const Obj &o : parse_entries(ss.str())) {
+        TuneKey key{o.s("arch"), o.i("prec", -1), o.i("dk", -1), o.i("kl", -1), o.i("wide", 0), o.i("nlog2", -1)};
+        TuneEntry e;
+        e.tv.split = o.i("split", 1);
+        e.tv.stage_mult = o.i("stage_mult", 2);
+        e.tv.dma = o.i("dma", 1);
+        e.ms = o.f("ms");
+        if (key.prec < 0 || key.dk < 0 || key.kl < 0 || key.nlog2 < 0) continue;
+        t.entries[key] = e;
+    }
+}
+
+void save_file_locked(const TuneTable &t, const std::string &path) {
+    const std::string tmp = path + ".tmp";
+    {
+        std::ofstream f(tmp);
+        if (!f) throw std::invalid_argument("[ncvis_amd] cannot write tuning cache " + path);
+        f << "{\n  \"format\": \"ncvis_amd kNN tile variants\",\n  \"version\": 1,\n  \"entries\": [";
+        bool first = true;
+        for (const auto &kv : t.entries) {
+            const TuneKey &k = kv.first;
+            const TuneEntry &e = kv.second;
+            char buf[512];
+            std::snprintf(buf, sizeof(buf),
+                          "%s\n    {\"arch\": \"%s\", \"prec\": %d, \"dk\": %d, \"kl\": %d, \"wide\": %d, \"nlog2\": %d, "
+                          "\"split\": %d, \"stage_mult\": %d, \"dma\": %d, \"ms\": %.5f}",
+                          first ? "" : ",", k.arch.c_str(), k.prec, k.dk, k.kl, k.wide, k.nlog2, e.tv.split,
+                          e.tv.stage_mult, e.tv.dma, e.ms);
+            f << buf;
+            first = false;
+        }
+        f << "\n  ]\n}\n";
+        if (!f) throw std::invalid_argument("[ncvis_amd] cannot write tuning cache " + path);
+    }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0)
+        throw std::invalid_argument("[ncvis_amd] cannot replace tuning cache " + path);
+}
+
+std::string env_path() {
+    const char *p = std::getenv("NCVIS_AMD_TUNE_CACHE");
+    return (p && *p) ? std::string(p) : std::string();
+}
+
+// uniform [0, 1) rows for timing a shape without data: pcg64(seed + row)
+__global__ void k_synthetic_rows(float *X, int64_t N, int D, uint64_t seed) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    U128 g = pcg_seed(seed + (uint64_t)i);
+    for (int j = 0; j < D; ++j) X[i * D + j] = uniform_unit_float(g);
+}
+
+}  // namespace
+
+bool tuned_variant(int prec, int DK, int KL, bool wide, int64_t N, TileVariant &out) {
+    TuneTable &t = table();
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (!t.env_loaded) {
+        t.env_loaded = true;
+        const std::string p = env_path();
+        if (!p.empty()) {
+            std::ifstream probe(p);
+            if (probe) load_file_locked(t, p);
+        }
+    }
+    if (t.entries.empty()) return false;
+    static const std::string arch = device_arch();
+    auto it = t.entries.find(TuneKey{arch, prec, DK, KL, wide ? 1 : 0, nlog2_of(N)});
+    if (it == t.entries.end()) return false;
+    out = it->second.tv;
+    return true;
+}
+
+}  // namespace ncvis_hip
+
+using namespace ncvis_hip;
+
+static thread_local std::string g_tune_error;
+
+extern "C" {
+
+const char *ncvis_autotune_last_error(void) { return g_tune_error.c_str(); }
+
+int ncvis_autotune_knn(const float *dX, size_t N, size_t D, size_t k, int dist, int precision, const char *cache_path,
+                       int *split_out, int *stage_mult_out, int *dma_out, double *ms_out) {
+    try {
+        if (N < 2 || D < 1 || k < 1 || k > N - 1) throw std::invalid_argument("[ncvis_amd] autotune: need 1 <= k <= N-1");
+        if (dist < DIST_L2 || dist > DIST_CORRELATION) throw std::invalid_argument("[ncvis_amd] autotune: bad distance");
+        if (precision < PREC_FP32 || precision > PREC_FP16)
+            throw std::invalid_argument("[ncvis_amd] autotune: bad precision");
+        hipStream_t st = nullptr;
+        NCVIS_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } sg{st};
+        DevBuf<float> synth;
+        if (!dX) {
+            synth.alloc(N * D);
+            k_synthetic_rows<<<cdiv((int64_t)N, 256), 256, 0, st>>>(synth.p, (int64_t)N, (int)D, 1234);
+            NCVIS_LAUNCH_CHECK();
+            dX = synth.p;
+        }
+        // candidate variants: every split up to the search's limit, times the precision's knobs
+        const TileVariant base = knn_variant((int64_t)N, (int)D, (int)k, dist, precision);
+        const bool wide = D > 256;
+        std::vector<TileVariant> cands;
+        for (int s = 1; s <= 4; s *= 2) {
+            TileVariant tv = base;
+            tv.split = s;
+            std::vector<int> sms{base.stage_mult}, dmas{base.dma};
+            if (precision != PREC_FP32 && !wide && k + 1 <= 16) sms = {1, 2};
+            if (precision == PREC_FP32 && !wide) dmas = {0, 1};
+            for (int sm : sms)
+                for (int dm : dmas) {
+                    tv.stage_mult = sm;
+                    tv.dma = dm;
+                    cands.push_back(tv);
+                }
+        }
+        size_t ws_bytes = 0;
+        for (const TileVariant &tv : cands)
+            ws_bytes = std::max(ws_bytes, knn_workspace_bytes_for((int64_t)N, (int)D, (int)k, dist, precision, tv));
+        DevBuf<unsigned char> ws(ws_bytes);
+        DevBuf<int32_t> inds(N * k);
+        DevBuf<float> dists(N * k);
+        DevBuf<int> err(1);
+        KnnEvents ev{};
+        NCVIS_HIP_CHECK(hipEventCreate(&ev.tiles_begin));
+        NCVIS_HIP_CHECK(hipEventCreate(&ev.tiles_end));
+        NCVIS_HIP_CHECK(hipEventCreate(&ev.rerank_end));
+        struct EvGuard {
+            KnnEvents &e;
+            ~EvGuard() {
+                (void)hipEventDestroy(e.tiles_begin);
+                (void)hipEventDestroy(e.tiles_end);
+                (void)hipEventDestroy(e.rerank_end);
+            }
+        } eg{ev};
+        TileVariant best = base;
+        double best_ms = 1e300;
+        std::vector<int> seen;
+        for (const TileVariant &cand : cands) {
+            // the layout clamps the split to what the search admits: time each effective variant once
+            const TileVariant tv = knn_effective_variant((int64_t)N, (int)D, (int)k, dist, precision, cand);
+            const int code = tv.split * 100 + tv.stage_mult * 10 + tv.dma;
+            if (std::find(seen.begin(), seen.end(), code) != seen.end()) continue;
+            seen.push_back(code);
+            double ms = 1e300;
+            for (int rep = 0; rep < 4; ++rep) {
+                knn_enqueue(dX, (int64_t)N, (int)D, (int)k, dist, precision, inds.p, dists.p, ws.p, err.p, st, &ev,
+                            &tv);
+                NCVIS_HIP_CHECK(hipStreamSynchronize(st));
+                float t = 0.f;
+                NCVIS_HIP_CHECK(hipEventElapsedTime(&t, ev.tiles_begin, ev.rerank_end));
+                if (rep > 0) ms = std::min(ms, (double)t);  // the first run warms the caches
+            }
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = tv;
+            }
+        }
+        // the entry's key: the search's padded width, list length and row band
+        const int DK = D <= 32 ? 1 : D <= 64 ? 2 : D <= 128 ? 4 : D <= 256 ? 8 : D <= 320 ? 10 : D <= 384 ? 12 : 16;
+        const int KL = (k + 1 <= 16) ? 16 : (k + 1 <= 32) ? 32 : 64;
+        TuneTable &t = table();
+        {
+            std::lock_guard<std::mutex> lk(t.mu);
+            t.env_loaded = true;  // an explicit tuning run takes precedence over a later automatic load
+            TuneEntry e;
+            e.tv = best;
+            e.ms = best_ms;
+            t.entries[TuneKey{device_arch(), precision, DK, KL, wide ? 1 : 0, nlog2_of((int64_t)N)}] = e;
+            std::string path = cache_path ? std::string(cache_path) : std::string();
+            if (path.empty() && cache_path) path = env_path();
+            if (!path.empty()) save_file_locked(t, path);
+        }
+        bump_tuning_generation();
+        if (split_out) *split_out = best.split;
+        if (stage_mult_out) *stage_mult_out = best.stage_mult;
+        if (dma_out) *dma_out = best.dma;
+        if (ms_out) *ms_out = best_ms;
+        g_tune_error.clear();
+        return NCVIS_OK;
+    } catch (const HipError &e) {
+        g_tune_error = e.what();
+        return NCVIS_EHIP;
+    } catch (const std::invalid_argument &e) {
+        g_tune_error = e.what();
+        return NCVIS_EINVAL;
+    } catch (const std::exception &e) {
+        g_tune_error = e.what();
+        return NCVIS_EINTERNAL;
+    }
+}
+
+int ncvis_load_tune_cache(const char *path) {
+    try {
+        TuneTable &t = table();
+        {
+            std::lock_guard<std::mutex> lk(t.mu);
+            t.env_loaded = true;
+            t.entries.clear();
+            if (path && *path) load_file_locked(t, path);
+        }
+        bump_tuning_generation();
+        g_tune_error.clear();
+        return NCVIS_OK;
+    } catch (const std::exception &e) {
+        g_tune_error = e.what();
+        return NCVIS_EINVAL;
+    }
+}
+
+int ncvis_tune_cache_size(void) {
+    TuneTable &t = table();
+    std::lock_guard<std::mutex> lk(t.mu);
+    return (int)t.entries.size();
+}
+
+}  // extern "C"

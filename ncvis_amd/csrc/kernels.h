@@ -20,10 +20,20 @@ struct Tuning {
     int64_t max_chains = 1 << 17;    // ... and at most this
     int knn_dma = -1;                // FP32 kNN tiles: stages by LDS DMA (1 or auto -1) or registers (0)
     int knn_stage_mult = 2;          // 16-bit kNN (k + 1 <= 16): LDS stage rows x1 or x2
+    int knn_split = 0;               // candidate split of untuned searches (0: by workgroup count)
 };
 Tuning &tuning();
 
 // ---- kNN (knn.hip) ---------------------------------------------------------------------
+// Result-invariant launch choices of the candidate tiles: the candidate split (grid.y ranges of
+// the candidate sweep, each keeping lists of its own, so small searches fill the GPU; every
+// range's lists go to the exact rerank), the 16-bit LDS stage multiplier and FP32 LDS DMA.
+struct TileVariant {
+    int split = 1;
+    int stage_mult = 2;
+    int dma = 1;
+};
+
 // Exact k-nearest-neighbour table in the reference's findKNN row format: for each row the k+1
 // smallest (distance, label) pairs are found, the smallest is dropped and the remaining k are
 // written farthest first. `prec` selects the candidate GEMM precision; the final distances are
@@ -41,30 +51,47 @@ struct KnnEvents {
 };
 // The same search queued on `st` with no host synchronisation and no allocation: ws holds
 // knn_workspace_bytes(N, D, k, dist, prec); a row left with fewer than k+1 finite candidates sets
-// kErrKnnCandidates in *d_err (which the caller zeroes).
+// kErrKnnCandidates in *d_err (which the caller zeroes). label_off is added to every written
+// neighbour index (tables of several data sets concatenated into one).
 size_t knn_workspace_bytes(int64_t N, int D, int k, int dist, int prec);
 void knn_enqueue(const float *dX, int64_t N, int D, int k, int dist, int prec, int32_t *d_inds, float *d_dists,
-                 void *ws, int *d_err, hipStream_t st, const KnnEvents *ev = nullptr);
+                 void *ws, int *d_err, hipStream_t st, const KnnEvents *ev = nullptr,
+                 const TileVariant *force = nullptr, int32_t label_off = 0);
+// Workspace of a search with a forced variant (the autotuner's measurements).
+size_t knn_workspace_bytes_for(int64_t N, int D, int k, int dist, int prec, const TileVariant &tv);
+// The variant a forced one becomes once the search's limits clamp it.
+TileVariant knn_effective_variant(int64_t N, int D, int k, int dist, int prec, const TileVariant &tv);
 
-// Candidate-tile launches on the packed matrix Xh [Npad, 32*DK] and offsets cn [Npad], writing the
-// two lane half-lists of KL ids per query to cand [N, 2*KL] (one translation unit each).
+// Process-wide generation of everything a fit plan depends on besides its handle (tuning knobs,
+// the autotuning table); bumped by every change.
+uint64_t tuning_generation();
+void bump_tuning_generation();
+
+// The variant the autotuning table holds for this search (false: none).
+bool tuned_variant(int prec, int DK, int KL, bool wide, int64_t N, TileVariant &out);
+// The variant knn_enqueue uses for this search.
+TileVariant knn_variant(int64_t N, int D, int k, int dist, int prec);
+
+// Candidate-tile launches on the packed matrix Xh [Npad, 32*DK] and offsets cn [Npad], writing,
+// for each of the `split` candidate ranges, the lane lists of KL ids per query to
+// cand [N, split, lists * KL] (one translation unit each).
 void knn_tiles16(int prec, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                 int32_t *cand, hipStream_t st);  // KL = 16 (knn_t16.hip)
+                 int32_t *cand, hipStream_t st, const TileVariant &tv);  // KL = 16 (knn_t16.hip)
 // KL = 32 or 64, one translation unit per GEMM precision / FP32 staging (knn_q1_*.hip)
 void knn_tiles_q1_f32dma(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                         int32_t *cand, hipStream_t st);
+                         int32_t *cand, hipStream_t st, int split);
 void knn_tiles_q1_f32(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                      int32_t *cand, hipStream_t st);
+                      int32_t *cand, hipStream_t st, int split);
 void knn_tiles_q1_bf16(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                       int32_t *cand, hipStream_t st);
+                       int32_t *cand, hipStream_t st, int split);
 void knn_tiles_q1_fp16(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                       int32_t *cand, hipStream_t st);
+                       int32_t *cand, hipStream_t st, int split);
 void knn_tiles_wide(int prec, int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn,
-                    float beta, int32_t *cand, hipStream_t st);  // 16-bit, DK 10/12/16 (knn_wide.hip)
+                    float beta, int32_t *cand, hipStream_t st, int split);  // 16-bit, DK 10/12/16 (knn_wide.hip)
 // FP32 rows of 257..512 elements (DK 10/12/16, KL 16/32): 16x16x4 tiles, four lists of KL per
-// query, cand [N, 4*KL], Npad a multiple of 64 (knn_wide_f32.hip)
+// query and range, Npad a multiple of 64 (knn_wide_f32.hip)
 void knn_tiles_wide_f32(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                        int32_t *cand, hipStream_t st);
+                        int32_t *cand, hipStream_t st, int split);
 
 // ---- symmetrize (symmetrize.hip) ---------------------------------------------------------
 // KNNTable::symmetrize at one thread, bit-exact, then flattened to CSR (= build_edges order).

@@ -264,7 +264,7 @@ template <int C>
 __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ Xp, int64_t N, int D, int is_l2,
                                                     int vec4, const int32_t *__restrict__ cand, int k,
                                                     int32_t *__restrict__ inds, float *__restrict__ dists,
-                                                    int *__restrict__ err) {
+                                                    int *__restrict__ err, int32_t label_off) {
     constexpr int G = 64 / C;
     const int lane = threadIdx.x & 63;
     const int c = lane % C;
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ Xp
     }
     if (valid && rank >= 1 && rank <= k) {
         if (ci == INT_MAX) atomicOr(err, kErrKnnCandidates);
-        inds[q * k + (k - rank)] = (ci == INT_MAX) ? 0 : ci;
+        inds[q * k + (k - rank)] = ((ci == INT_MAX) ? 0 : ci) + label_off;
         dists[q * k + (k - rank)] = dv;
     }
 }
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float *__restrict__ Xp
 __global__ __launch_bounds__(256) void k_knn_rerank128(const float *__restrict__ Xp, int64_t N, int D, int is_l2,
                                                        int vec4, const int32_t *__restrict__ cand, int k,
                                                        int32_t *__restrict__ inds, float *__restrict__ dists,
-                                                       int *__restrict__ err) {
+                                                       int *__restrict__ err, int32_t label_off) {
     const int lane = threadIdx.x & 63;
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const bool valid = q < N;
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) void k_knn_rerank128(const float *__restrict__
     for (int u = 0; u < 2; ++u) {
         if (valid && rank[u] >= 1 && rank[u] <= k) {
             if (ci[u] == INT_MAX) atomicOr(err, kErrKnnCandidates);
-            inds[q * k + (k - rank[u])] = (ci[u] == INT_MAX) ? 0 : ci[u];
+            inds[q * k + (k - rank[u])] = ((ci[u] == INT_MAX) ? 0 : ci[u]) + label_off;
             dists[q * k + (k - rank[u])] = dv[u];
         }
     }
@@ -385,7 +385,8 @@ __device__ __noinline__ int heap_offer(float *hd, int *hi, int cnt, int K1, floa
 // query go through a max-heap, then the smallest is dropped and the rest written farthest first.
 __global__ __launch_bounds__(256) void k_knn_generic(const float *__restrict__ Xp, int64_t N, int D, int is_l2,
                                                      int k, float *__restrict__ hd_, int *__restrict__ hi_,
-                                                     int32_t *__restrict__ inds, float *__restrict__ dists) {
+                                                     int32_t *__restrict__ inds, float *__restrict__ dists,
+                                                     int32_t label_off) {
     constexpr int TC = 64, DC = 32;
     __shared__ float lq[256][DC + 1];
     __shared__ __align__(16) float lc[DC][TC];
@@ -474,7 +475,7 @@ __global__ __launch_bounds__(256) void k_knn_generic(const float *__restrict__ X
         hi[pos] = vi;
     }
     for (int j = 0; j < k; ++j) {
-        inds[q * k + j] = hi[K1 - 1 - j];
+        inds[q * k + j] = hi[K1 - 1 - j] + label_off;
         dists[q * k + j] = hd[K1 - 1 - j];
     }
 }
@@ -484,12 +485,21 @@ struct KnnLayout {
     bool tiled, wide, wide32;
     int DK, Dpad, QB, KL, lists, C, esz, nb;
     int64_t Npad;
+    TileVariant tv;
     size_t xp, part, mean, cs, xh, cn, cand, hd, hi, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-static KnnLayout knn_layout(int64_t N, int D, int k, int dist, int prec) {
+// Largest candidate split a search admits: every range keeps at least 128 candidate rows (the
+// largest LDS stage) and the rerank takes at most 128 candidates per query.
+static int max_split(int64_t Npad, int lists, int KL) {
+    int s = 1;
+    while (s < 4 && Npad / (2 * s) >= 128 && lists * KL * 2 * s <= 128) s *= 2;
+    return s;
+}
+
+static KnnLayout knn_layout(int64_t N, int D, int k, int dist, int prec, const TileVariant *force = nullptr) {
     KnnLayout L{};
     // rows up to 256 elements: 32x32 MFMA tiles in every precision (k + 1 <= 64); rows of
     // 257..512: one query set per wave in the 16-bit GEMMs, 16x16x4 tiles in FP32 (k + 1 <= 32);
@@ -503,8 +513,27 @@ static KnnLayout knn_layout(int64_t N, int D, int k, int dist, int prec) {
     L.QB = (L.wide || L.KL >= 32 || prec == PREC_FP32) ? 1 : 2;
     const int64_t qtile = L.wide32 ? 64 : 4 * 32 * L.QB;
     L.Npad = ((N + qtile - 1) / qtile) * qtile;
-    L.lists = L.wide32 ? 4 : 2;  // candidate lists per query
-    L.C = L.lists * L.KL;        // candidates per query for the rerank: 32, 64 or 128
+    L.lists = L.wide32 ? 4 : 2;  // candidate lists per query and candidate range
+    if (L.tiled) {
+        const int smax = max_split(L.Npad, L.lists, L.KL);
+        if (force) {
+            L.tv = *force;
+        } else if (!tuned_variant(prec, L.DK, L.KL, L.wide, N, L.tv)) {
+            // untuned: split the candidates until the launch has about 1024 workgroups
+            const Tuning &tu = tuning();
+            L.tv.stage_mult = tu.knn_stage_mult;
+            L.tv.dma = tu.knn_dma != 0 ? 1 : 0;
+            const int64_t blocks = L.Npad / qtile;
+            L.tv.split = 1;
+            if (tu.knn_split > 0)
+                L.tv.split = tu.knn_split;
+            else
+                while (L.tv.split < smax && blocks * L.tv.split < 1024) L.tv.split *= 2;
+        }
+        if (L.tv.split < 1) L.tv.split = 1;
+        if (L.tv.split > smax) L.tv.split = smax;
+    }
+    L.C = L.lists * L.KL * L.tv.split;  // candidates per query for the rerank: 32, 64 or 128
     L.esz = (prec == PREC_FP32) ? 4 : 2;
     L.nb = (int)std::min<int64_t>(512, std::max<int64_t>(1, cdiv(N, 256)));
     size_t o = 0;
@@ -533,6 +562,16 @@ size_t knn_workspace_bytes(int64_t N, int D, int k, int dist, int prec) {
     return knn_layout(N, D, k, dist, prec).total;
 }
 
+TileVariant knn_variant(int64_t N, int D, int k, int dist, int prec) { return knn_layout(N, D, k, dist, prec).tv; }
+
+size_t knn_workspace_bytes_for(int64_t N, int D, int k, int dist, int prec, const TileVariant &tv) {
+    return knn_layout(N, D, k, dist, prec, &tv).total;
+}
+
+TileVariant knn_effective_variant(int64_t N, int D, int k, int dist, int prec, const TileVariant &tv) {
+    return knn_layout(N, D, k, dist, prec, &tv).tv;
+}
+
 // cs [D + 1]: per-column offsets (0 unless centring) and the scale, all on the device.
 static void gemm_input_transform(const float *Xp, int64_t N, int D, int center, int nb, double *part, double *mean,
                                  float *cs, hipStream_t st) {
@@ -548,11 +587,13 @@ static void gemm_input_transform(const float *Xp, int64_t N, int D, int center, 
 }
 
 void knn_enqueue(const float *dX, int64_t N, int D, int k, int dist, int prec, int32_t *d_inds, float *d_dists,
-                 void *ws, int *d_err, hipStream_t st, const KnnEvents *ev) {
+                 void *ws, int *d_err, hipStream_t st, const KnnEvents *ev, const TileVariant *force,
+                 int32_t label_off) {
     if (N < 2 || D < 1 || k < 1 || k > N - 1) throw HipError("[ncvis_amd] knn: invalid sizes");
     if (N > (int64_t)INT_MAX - 512) throw HipError("[ncvis_amd] knn: more than 2^31 points");
     if (prec < PREC_FP32 || prec > PREC_FP16) throw HipError("[ncvis_amd] knn: unknown precision");
-    const KnnLayout L = knn_layout(N, D, k, dist, prec);
+    const KnnLayout L = knn_layout(N, D, k, dist, prec, force);
+    const int S = L.tv.split;
     unsigned char *w = (unsigned char *)ws;
     const int is_l2 = (dist == DIST_L2);
     const float *Xp = dX;
@@ -588,34 +629,37 @@ void knn_enqueue(const float *dX, int64_t N, int D, int k, int dist, int prec, i
         const float beta = is_l2 ? -2.0f : -1.0f;
         if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->tiles_begin, st));
         if (L.wide32)
-            knn_tiles_wide_f32(KL, DK, N, Npad, xh, cn, beta, cand, st);
+            knn_tiles_wide_f32(KL, DK, N, Npad, xh, cn, beta, cand, st, S);
         else if (L.wide)
-            knn_tiles_wide(prec, KL, DK, N, Npad, xh, cn, beta, cand, st);
+            knn_tiles_wide(prec, KL, DK, N, Npad, xh, cn, beta, cand, st, S);
         else if (KL == 16)
-            knn_tiles16(prec, DK, N, Npad, xh, cn, beta, cand, st);
-        else if (prec == PREC_FP32 && tuning().knn_dma != 0)
-            knn_tiles_q1_f32dma(KL, DK, N, Npad, xh, cn, beta, cand, st);
+            knn_tiles16(prec, DK, N, Npad, xh, cn, beta, cand, st, L.tv);
+        else if (prec == PREC_FP32 && L.tv.dma != 0)
+            knn_tiles_q1_f32dma(KL, DK, N, Npad, xh, cn, beta, cand, st, S);
         else if (prec == PREC_FP32)
-            knn_tiles_q1_f32(KL, DK, N, Npad, xh, cn, beta, cand, st);
+            knn_tiles_q1_f32(KL, DK, N, Npad, xh, cn, beta, cand, st, S);
         else if (prec == PREC_BF16)
-            knn_tiles_q1_bf16(KL, DK, N, Npad, xh, cn, beta, cand, st);
+            knn_tiles_q1_bf16(KL, DK, N, Npad, xh, cn, beta, cand, st, S);
         else
-            knn_tiles_q1_fp16(KL, DK, N, Npad, xh, cn, beta, cand, st);
+            knn_tiles_q1_fp16(KL, DK, N, Npad, xh, cn, beta, cand, st, S);
         NCVIS_LAUNCH_CHECK();
         if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->tiles_end, st));
         const int vec4 = (D % 4 == 0) && ((uintptr_t)Xp % 16 == 0);
         if (L.C == 32)
-            k_knn_rerank<32><<<cdiv(N, 8), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand, k, d_inds, d_dists, d_err);
+            k_knn_rerank<32><<<cdiv(N, 8), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand, k, d_inds, d_dists, d_err,
+                                                            label_off);
         else if (L.C == 64)
-            k_knn_rerank<64><<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand, k, d_inds, d_dists, d_err);
+            k_knn_rerank<64><<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand, k, d_inds, d_dists, d_err,
+                                                            label_off);
         else
-            k_knn_rerank128<<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand, k, d_inds, d_dists, d_err);
+            k_knn_rerank128<<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand, k, d_inds, d_dists, d_err,
+                                                            label_off);
         NCVIS_LAUNCH_CHECK();
         if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->rerank_end, st));
     } else {
         if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->tiles_begin, st));
         k_knn_generic<<<cdiv(N, 256), 256, 0, st>>>(Xp, N, D, is_l2, k, (float *)(w + L.hd), (int *)(w + L.hi),
-                                                    d_inds, d_dists);
+                                                    d_inds, d_dists, label_off);
         NCVIS_LAUNCH_CHECK();
         if (ev) {
             NCVIS_HIP_CHECK(hipEventRecord(ev->tiles_end, st));

@@ -4,8 +4,8 @@
 namespace ncvis_hip {
 
 void knn_tiles_q1_f32(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                        int32_t *cand, hipStream_t st) {
-    launch_tiles_q1<PREC_FP32, 0>(KL, DK, N, Npad, Xh, cn, beta, cand, st);
+                        int32_t *cand, hipStream_t st, int split) {
+    launch_tiles_q1<PREC_FP32, 0>(KL, DK, N, Npad, Xh, cn, beta, cand, st, split);
 }
 
 }  // namespace ncvis_hip

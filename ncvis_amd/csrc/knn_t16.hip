@@ -4,11 +4,11 @@
 namespace ncvis_hip {
 
 void knn_tiles16(int prec, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                 int32_t *cand, hipStream_t st) {
+                 int32_t *cand, hipStream_t st, const TileVariant &tv) {
     switch (prec) {
-        case PREC_FP32: launch_tiles16<PREC_FP32>(DK, N, Npad, Xh, cn, beta, cand, st); break;
-        case PREC_BF16: launch_tiles16<PREC_BF16>(DK, N, Npad, Xh, cn, beta, cand, st); break;
-        default: launch_tiles16<PREC_FP16>(DK, N, Npad, Xh, cn, beta, cand, st); break;
+        case PREC_FP32: launch_tiles16<PREC_FP32>(DK, N, Npad, Xh, cn, beta, cand, st, tv); break;
+        case PREC_BF16: launch_tiles16<PREC_BF16>(DK, N, Npad, Xh, cn, beta, cand, st, tv); break;
+        default: launch_tiles16<PREC_FP16>(DK, N, Npad, Xh, cn, beta, cand, st, tv); break;
     }
 }
 

@@ -127,13 +127,17 @@ __global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_,
         thr[qb] = INFINITY;
     }
 
-    const int64_t nstages = Npad / CB;
+    // candidate split: block row y of S sweeps stages [sb, se) and keeps lists of its own
+    const int S = (int)gridDim.y, ysp = (int)blockIdx.y;
+    const int64_t total = Npad / CB;
+    const int64_t sb = total * ysp / S, se = total * (ysp + 1) / S;
     const bool cn_lane = threadIdx.x < CB / 4;  // these threads carry the stage's cn (float4 each)
     uint4 stage[NV];
     float4 stage_cn = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(Xh + (int64_t)(v * 256 + threadIdx.x) * 16);
-    if (cn_lane) stage_cn = *(const float4 *)(cn + 4 * threadIdx.x);
+    for (int v = 0; v < NV; ++v)
+        stage[v] = *(const uint4 *)(Xh + sb * CB * (int64_t)ROWB + (int64_t)(v * 256 + threadIdx.x) * 16);
+    if (cn_lane) stage_cn = *(const float4 *)(cn + sb * CB + 4 * threadIdx.x);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         int u = v * 256 + threadIdx.x;
@@ -143,11 +147,11 @@ __global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_,
     if (cn_lane) *(float4 *)(&ldc[0][4 * threadIdx.x]) = stage_cn;
     __syncthreads();
 
-    for (int64_t st = 0; st < nstages; ++st) {
-        const int buf = (int)(st & 1);
+    for (int64_t st = sb; st < se; ++st) {
+        const int buf = (int)((st - sb) & 1);
         {
             // next stage (the last iteration reloads its own stage; the copy is never read)
-            const int64_t nx = (st + 1 < nstages) ? st + 1 : st;
+            const int64_t nx = (st + 1 < se) ? st + 1 : st;
             const unsigned char *nb = Xh + nx * CB * (int64_t)ROWB;
 #pragma unroll
             for (int v = 0; v < NV; ++v) stage[v] = *(const uint4 *)(nb + (int64_t)(v * 256 + threadIdx.x) * 16);
@@ -227,7 +231,7 @@ __global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_,
     for (int qb = 0; qb < QB; ++qb) {
         int64_t q = q0 + qb * 32 + col;
         if (q < N) {
-            int32_t *o = cand + q * (2 * KL) + h * KL;
+            int32_t *o = cand + q * (2 * KL * S) + ysp * (2 * KL) + h * KL;
 #pragma unroll
             for (int p = 0; p < KL; ++p) o[p] = li[qb][p];
         }
@@ -326,8 +330,10 @@ __global__ __launch_bounds__(256) void k_knn_tiles_dma(const void *__restrict__ 
         thr[qb] = INFINITY;
     }
 
-    const int64_t nstages = Npad / CB;
-    const int64_t last = nstages - 1;
+    const int S = (int)gridDim.y, ysp = (int)blockIdx.y;
+    const int64_t total = Npad / CB;
+    const int64_t sb = total * ysp / S, se = total * (ysp + 1) / S;
+    const int64_t last = se - 1;
     const int swz = col & (CHUNKS - 1);  // rows rb*32 + col share col's swizzle (CHUNKS <= 32 divides 32)
 
 #define NCVIS_DMA(STG, LB, LC) \
@@ -396,13 +402,13 @@ __global__ __launch_bounds__(256) void k_knn_tiles_dma(const void *__restrict__ 
         NCVIS_DMA_COMPUTE(LB, LC, ST);                                                                 \
     } while (0)
 
-    NCVIS_DMA(0, lds0, ldc0);
-    NCVIS_DMA(1, lds1, ldc1);
-    for (int64_t st = 0; st < nstages; st += 3) {
+    NCVIS_DMA(sb, lds0, ldc0);
+    NCVIS_DMA(sb + 1, lds1, ldc1);
+    for (int64_t st = sb; st < se; st += 3) {
         NCVIS_DMA_STEP(st, lds0, ldc0, lds2, ldc2);
-        if (st + 1 >= nstages) break;
+        if (st + 1 >= se) break;
         NCVIS_DMA_STEP(st + 1, lds1, ldc1, lds0, ldc0);
-        if (st + 2 >= nstages) break;
+        if (st + 2 >= se) break;
         NCVIS_DMA_STEP(st + 2, lds2, ldc2, lds1, ldc1);
     }
     NCVIS_WAIT_VMCNT(0);
@@ -414,7 +420,7 @@ __global__ __launch_bounds__(256) void k_knn_tiles_dma(const void *__restrict__ 
     for (int qb = 0; qb < QB; ++qb) {
         int64_t q = q0 + qb * 32 + col;
         if (q < N) {
-            int32_t *o = cand + q * (2 * KL) + h * KL;
+            int32_t *o = cand + q * (2 * KL * S) + ysp * (2 * KL) + h * KL;
 #pragma unroll
             for (int p = 0; p < KL; ++p) o[p] = li[qb][p];
         }
@@ -426,11 +432,10 @@ __global__ __launch_bounds__(256) void k_knn_tiles_dma(const void *__restrict__ 
 // FP32: one set per wave, LDS-DMA stages by default (knn_dma = 0: register-staged).
 template <int P>
 static void launch_tiles16(int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                           int32_t *cand, hipStream_t st) {
-    const Tuning &tu = tuning();
+                           int32_t *cand, hipStream_t st, const TileVariant &tv) {
     if constexpr (P == PREC_FP32) {
-        dim3 grid((unsigned)(Npad / (4 * 32)));
-        if (tu.knn_dma != 0) {
+        dim3 grid((unsigned)(Npad / (4 * 32)), (unsigned)tv.split);
+        if (tv.dma != 0) {
             switch (DK) {
                 case 1: k_knn_tiles_dma<P, 1, 16, 1><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
                 case 2: k_knn_tiles_dma<P, 1, 16, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
@@ -448,8 +453,8 @@ static void launch_tiles16(int DK, int64_t N, int64_t Npad, const void *Xh, cons
             }
         }
     } else {
-        dim3 grid((unsigned)(Npad / (4 * 32 * 2)));
-        if (tu.knn_stage_mult == 2) {
+        dim3 grid((unsigned)(Npad / (4 * 32 * 2)), (unsigned)tv.split);
+        if (tv.stage_mult == 2) {
             switch (DK) {
                 case 1: k_knn_tiles<P, 2, 16, 1, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
                 case 2: k_knn_tiles<P, 2, 16, 2, 2><<<grid, 256, 0, st>>>(Xh, cn, N, Npad, beta, cand); break;
@@ -474,8 +479,8 @@ static void launch_tiles16(int DK, int64_t N, int64_t Npad, const void *Xh, cons
 // DMA = 0: register-staged stages.
 template <int P, int DMA>
 static void launch_tiles_q1(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                            int32_t *cand, hipStream_t st) {
-    dim3 grid((unsigned)(Npad / (4 * 32)));
+                            int32_t *cand, hipStream_t st, int split) {
+    dim3 grid((unsigned)(Npad / (4 * 32)), (unsigned)split);
 #define NCVIS_Q1(KLV, DKV)                                                                            \
     do {                                                                                             \
         if constexpr (DMA)                                                                           \
@@ -509,8 +514,8 @@ static void launch_tiles_wide_kl(int DK, dim3 grid, const void *Xh, const float 
 
 template <int P>
 static void launch_tiles_wide(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                              int32_t *cand, hipStream_t st) {
-    dim3 grid((unsigned)(Npad / (4 * 32)));
+                              int32_t *cand, hipStream_t st, int split) {
+    dim3 grid((unsigned)(Npad / (4 * 32)), (unsigned)split);
     if (KL == 16)
         launch_tiles_wide_kl<P, 16>(DK, grid, Xh, cn, N, Npad, beta, cand, st);
     else

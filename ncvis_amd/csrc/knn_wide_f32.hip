@@ -87,8 +87,10 @@ __global__ __launch_bounds__(256) void k_knn_tiles_f32w(const float *__restrict_
     }
     float thr = INFINITY;
 
-    const int64_t nstages = Npad / CB;
-    const int64_t last = nstages - 1;
+    const int S = (int)gridDim.y, ysp = (int)blockIdx.y;  // candidate split (stages [sb, se))
+    const int64_t total = Npad / CB;
+    const int64_t sb = total * ysp / S, se = total * (ysp + 1) / S;
+    const int64_t last = se - 1;
 
 #define NCVIS_WDMA(STG, LB, LC) \
     dma_stage_wide<CB, CHUNKS, NV>(Xh, cn, (STG) < last ? (STG) : last, LB, LC[wave], wave, lane)
@@ -130,13 +132,13 @@ __global__ __launch_bounds__(256) void k_knn_tiles_f32w(const float *__restrict_
         NCVIS_WCOMPUTE(LB, LC, ST);       \
     } while (0)
 
-    NCVIS_WDMA(0, lds0, ldc0);
-    NCVIS_WDMA(1, lds1, ldc1);
-    for (int64_t st = 0; st < nstages; st += 3) {
+    NCVIS_WDMA(sb, lds0, ldc0);
+    NCVIS_WDMA(sb + 1, lds1, ldc1);
+    for (int64_t st = sb; st < se; st += 3) {
         NCVIS_WSTEP(st, lds0, ldc0, lds2, ldc2);
-        if (st + 1 >= nstages) break;
+        if (st + 1 >= se) break;
         NCVIS_WSTEP(st + 1, lds1, ldc1, lds0, ldc0);
-        if (st + 2 >= nstages) break;
+        if (st + 2 >= se) break;
         NCVIS_WSTEP(st + 2, lds2, ldc2, lds1, ldc1);
     }
     NCVIS_WAIT_VMCNT(0);
@@ -146,16 +148,16 @@ __global__ __launch_bounds__(256) void k_knn_tiles_f32w(const float *__restrict_
 
     const int64_t q = q0 + col;
     if (q < N) {
-        int32_t *o = cand + q * (4 * KL) + g * KL;
+        int32_t *o = cand + q * (4 * KL * S) + ysp * (4 * KL) + g * KL;
 #pragma unroll
         for (int p = 0; p < KL; ++p) o[p] = li[p];
     }
 }
 
 void knn_tiles_wide_f32(int KL, int DK, int64_t N, int64_t Npad, const void *Xh, const float *cn, float beta,
-                        int32_t *cand, hipStream_t st) {
+                        int32_t *cand, hipStream_t st, int split) {
     if (Npad % 64 != 0) throw HipError("[ncvis_amd] wide FP32 tiles need Npad % 64 == 0");
-    const dim3 grid((unsigned)(Npad / 64));
+    const dim3 grid((unsigned)(Npad / 64), (unsigned)split);
     const float *X = (const float *)Xh;
 #define NCVIS_W(KLV, DKV) k_knn_tiles_f32w<KLV, DKV><<<grid, 256, 0, st>>>(X, cn, N, Npad, beta, cand)
     if (KL == 16) {

@@ -127,3 +127,23 @@ def test_exact_threads_is_a_keyword():
     sig = inspect.signature(ncvis_amd.NCVis.__init__)
     assert sig.parameters["exact_threads"].default is False
     assert sig.parameters["n_threads"].default == -1
+
+
+def test_tune_cache_reader(tmp_path):
+    # the JSON cache of the kNN autotuner loads without a GPU; malformed files are refused
+    from ncvis_amd import abi
+    good = tmp_path / "tiles.json"
+    good.write_text('''{"format": "ncvis_amd kNN tile variants", "version": 1, "entries": [
+      {"arch": "gfx950", "prec": 2, "dk": 4, "kl": 16, "wide": 0, "nlog2": 14, "split": 4, "stage_mult": 2, "dma": 1, "ms": 0.5},
+      {"arch": "gfx950", "prec": 0, "dk": 2, "kl": 32, "wide": 0, "nlog2": 20, "split": 1, "stage_mult": 2, "dma": 0, "ms": 9.0}
+    ]}''')
+    try:
+        assert abi.load_tune_cache(good) == 2
+        bad = tmp_path / "bad.json"
+        bad.write_text('{"version": 1}')
+        with pytest.raises(RuntimeError, match="entries"):
+            abi.load_tune_cache(bad)
+        with pytest.raises(RuntimeError, match="cannot read"):
+            abi.load_tune_cache(tmp_path / "missing.json")
+    finally:
+        assert abi.load_tune_cache(None) == 0

@@ -403,10 +403,12 @@ def test_fit_transform_handle_device_api(torch_cuda):
 
 
 # ------------------------------------------------------------------------------ tuning knobs
-_KNN_DEFAULTS = {"knn_stage_mult": 2, "knn_dma": -1}
+_KNN_DEFAULTS = {"knn_stage_mult": 2, "knn_dma": -1, "knn_split": 0}
 
 
-@pytest.mark.parametrize("knobs", [{"knn_stage_mult": 1}, {"knn_dma": 0}, {"knn_dma": 0, "knn_stage_mult": 1}])
+@pytest.mark.parametrize("knobs", [{"knn_stage_mult": 1}, {"knn_dma": 0}, {"knn_dma": 0, "knn_stage_mult": 1},
+                                   {"knn_split": 1}, {"knn_split": 2}, {"knn_split": 4},
+                                   {"knn_split": 4, "knn_dma": 0}])
 @pytest.mark.parametrize("precision", ["fp16", "fp32"])
 def test_knn_tuning_knobs_keep_the_table(torch_cuda, knobs, precision):
     # every kernel variant a knob selects gives the exact table on integer data
@@ -443,3 +445,29 @@ def test_sgd_tuning_knobs_run(torch_cuda, knobs):
             abi.tune(key, v)
     assert np.all(np.isfinite(Y)) and np.isfinite(q)
     assert Y.std() > 0.5 * Y0.std()
+
+
+def test_knn_autotune_cache_round_trip(torch_cuda, tmp_path):
+    # the autotuner times the tile variants of a shape, keeps the fastest and writes the JSON
+    # cache; the table it selects still gives the exact table on integer data, and the cache
+    # loads back into a fresh table
+    from ncvis_amd import abi
+    import json
+    torch = torch_cuda
+    _say("knn autotune")
+    X, _ = synthetic.digits_like(N=6000, D=64, seed=31)
+    dX = _dev(torch, X)
+    path = tmp_path / "tiles.json"
+    try:
+        best = abi.autotune_knn(dX.data_ptr(), 6000, 64, 15, "euclidean", "fp16", cache_path=path)
+        assert best["split"] in (1, 2, 4) and best["stage_mult"] in (1, 2) and best["ms"] > 0
+        doc = json.loads(path.read_text())
+        assert len(doc["entries"]) == 1 and doc["entries"][0]["split"] == best["split"]
+        gi, gd = gpu_knn(torch, X, 15, "euclidean", "fp16")
+        oi, od = oracle.knn_bruteforce(oracle.preprocess(X), 15)
+        assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+        synth = abi.autotune_knn(None, 20000, 128, 15, "euclidean", "fp32", cache_path=path)
+        assert synth["dma"] in (0, 1)
+        assert abi.load_tune_cache(path) == 2
+    finally:
+        abi.load_tune_cache(None)
