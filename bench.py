@@ -210,6 +210,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle legs (parity, CPU baseline)")
     ap.add_argument("--no-companions", action="store_true", help="skip the other-precision fits")
     ap.add_argument("--no-autotune", action="store_true", help="batch64: keep the default tile variants")
+    ap.add_argument("--fit-only", action="store_true",
+                    help="time the fits and print the stage times only (short runs under a profiler)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the fit path runs on a single MI355X in this build "
@@ -263,6 +265,10 @@ def main():
     Q_gpu = s_last["Q"]
     assert np.all(np.isfinite(Y)), "non-finite embedding"
     E = s_last["n_edges"]
+    if args.fit_only:
+        print(json.dumps({"workload": w["desc"], "fit_s": round(sec, 4), "steps": args.steps,
+                          "stage_ms": {k_: round(v, 3) for k_, v in stage.items()}, "n_edges": E}), flush=True)
+        return
 
     # ---- kNN tables of all precisions vs float64 exact search (outside the timed region)
     rng = np.random.default_rng(1)
@@ -322,6 +328,15 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": traffic.get("k_sgd_epoch"),
                 "launch_ms": stage["sgd_kernel"] / n_epochs}
     roof_sgd["frac"] = roof_sgd["achieved"] / roof_sgd["peak"]
+    # the epoch's real bound: one random 8-byte-row load + device-scope 64-bit CAS per sample,
+    # calibrated on a table of Y's size with the epoch's thread count
+    if s_last["chains"] >= 256:
+        cal = abi.calibrate_rmw(N, (s_last["chains"] // 256) * 256, 512)
+        samples_per_s = n_samples / sgd_s
+        roof_sgd["rmw_ceiling"] = {"samples_per_s": samples_per_s,
+                                   "load_cas_rows_per_s": cal["load_cas_rows_per_s"],
+                                   "load_rows_per_s": cal["load_rows_per_s"],
+                                   "frac": samples_per_s / cal["load_cas_rows_per_s"]}
     dominant = roof_knn if knn_tiles_s >= sgd_s else roof_sgd
 
     # ---- the host-buffer entry point (X and Y cross PCIe): reported beside, never as `value`

@@ -137,6 +137,14 @@ int ncvis_load_tune_cache(const char *path);
 int ncvis_tune_cache_size(void);
 const char *ncvis_autotune_last_error(void);
 
+/* ---- measurement (no reference counterpart) ----
+ * The many-chain SGD epoch is bound by random 8-byte-row loads each followed by a device-scope
+ * 64-bit compare-and-swap. ncvis_calibrate_rmw measures that pattern's own ceiling: `threads`
+ * threads (multiple of 256) each load `iters` pcg64-random rows of a table of `rows` 8-byte
+ * entries, once read-only and once with a compare-and-swap update; outputs rows per second. */
+int ncvis_calibrate_rmw(size_t rows, int64_t threads, int iters, double *load_rows_per_s, double *cas_rows_per_s);
+const char *ncvis_calibrate_last_error(void);
+
 /* ncvis_optimize_device without any device allocation. No counterpart in the reference: it is
  * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc). Same arguments and results as ncvis_optimize_device;

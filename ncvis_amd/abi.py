@@ -18,6 +18,7 @@ EXPORTS = (
     "ncvis_symmetrize_device", "ncvis_init_embedding_device", "ncvis_optimize_device",
     "ncvis_nce_loss_device", "ncvis_tune", "ncvis_fit_transform_many", "ncvis_optimize_device_noalloc",
     "ncvis_autotune_knn", "ncvis_load_tune_cache", "ncvis_tune_cache_size", "ncvis_autotune_last_error",
+    "ncvis_calibrate_rmw", "ncvis_calibrate_last_error",
 )
 
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
@@ -66,6 +67,10 @@ def lib():
                                          C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                          C.POINTER(C.c_double)]
         L.ncvis_load_tune_cache.argtypes = [C.c_char_p]
+        L.ncvis_calibrate_rmw.argtypes = [C.c_size_t, C.c_int64, C.c_int, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]
+        L.ncvis_calibrate_last_error.argtypes = []
+        L.ncvis_calibrate_last_error.restype = C.c_char_p
         L.ncvis_tune_cache_size.argtypes = []
         L.ncvis_autotune_last_error.argtypes = []
         L.ncvis_autotune_last_error.restype = C.c_char_p
@@ -73,7 +78,8 @@ def lib():
                                             vp, C.c_float, C.c_float, C.c_float, vp, vp, vp, vp]
         for name in EXPORTS:
             f = getattr(L, name)
-            if name not in ("ncvis_destroy", "ncvis_last_error", "ncvis_get_last_error", "ncvis_autotune_last_error"):
+            if name not in ("ncvis_destroy", "ncvis_last_error", "ncvis_get_last_error", "ncvis_autotune_last_error",
+                            "ncvis_calibrate_last_error"):
                 f.restype = C.c_int
         _lib = L
     return _lib
@@ -100,6 +106,14 @@ def autotune_knn(dX, N: int, D: int, k: int, distance: str = "euclidean", precis
     if rc != 0:
         raise RuntimeError(lib().ncvis_autotune_last_error().decode())
     return {"split": sp.value, "stage_mult": sm.value, "dma": dm.value, "ms": ms.value}
+
+
+def calibrate_rmw(rows: int, threads: int, iters: int = 512):
+    """Random 8-byte-row load and load + 64-bit CAS rates (rows/s) on a table of `rows` entries."""
+    lo, ca = C.c_double(), C.c_double()
+    if lib().ncvis_calibrate_rmw(rows, threads, iters, C.byref(lo), C.byref(ca)) != 0:
+        raise RuntimeError(lib().ncvis_calibrate_last_error().decode())
+    return {"load_rows_per_s": lo.value, "load_cas_rows_per_s": ca.value}
 
 
 def load_tune_cache(path=None) -> int:
@@ -206,6 +220,7 @@ class Handle:
                 "Q": q.value, "chains": c.value}
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().ncvis_destroy(self._h)
+        # at interpreter shutdown the module globals may already be gone: nothing left to free then
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.ncvis_destroy(self._h)
             self._h = None
