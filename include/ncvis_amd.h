@@ -62,7 +62,9 @@ const char *ncvis_last_error(const ncvis_handle *h);
  *   "atomic_updates"  1: Hogwild Y updates with atomics (default when more than one chain runs),
  *                     0: plain read-modify-write stores like the reference
  *   "exact_threads"   1: run exactly n_threads chains (default 0: GPU-sized chain count)
- *   "chains"          override the SGD chain count (0 = none) */
+ *   "chains"          override the SGD chain count (0 = none)
+ *   "graph"           1: ncvis_fit_transform_many captures its work in a hipGraph and replays it
+ *                     (default), 0: queues it eagerly on every call */
 int ncvis_set_option(ncvis_handle *h, const char *key, double value);
 
 /* Process-wide tuning knobs (no reference counterpart). None changes a single-chain fit; the kNN
@@ -70,7 +72,8 @@ int ncvis_set_option(ncvis_handle *h, const char *key, double value);
  * "sgd_fast_math" (0/1), "edges_per_chain" (auto chain count = E / this, at least
  * min(E / 16, "min_chains"), at most "max_chains"),
  * "knn_dma" (FP32 tiles: -1 auto = 1 LDS DMA, 0 register-staged), "knn_stage_mult" (16-bit
- * tiles: 1 or 2 x the LDS stage rows). */
+ * tiles: 1 or 2 x the LDS stage rows), "knn_split" (candidate ranges per query block: 0 auto,
+ * else 1, 2 or 4; each range keeps lists of its own, so the table is unchanged). */
 int ncvis_tune(const char *key, double value);
 
 /* ncvis::NCVis::fit_transform: X [N, D] and Y [N, d] are host, row-major float32. */
@@ -82,10 +85,13 @@ int ncvis_fit_transform_device(ncvis_handle *h, const float *dX, size_t N, size_
 /* Many independent data sets in one call, each fitted with this handle's parameters exactly as
  * ncvis_fit_transform_device fits it (the reference's fit_transform, src/ncvis.cpp:354-459, takes
  * one data set; this batched form has no counterpart there). dX[i] [N[i], D[i]] and dY[i] [N[i], d]
- * are device pointers. Data sets run n_streams at a time, each on a stream of its own driven by
- * a host worker thread, so small data sets fill the GPU together. Blocking: waits for `stream`
- * (where the inputs were produced) first and returns when every fit is done. An error names the
- * first failing data set; the handle's stage times are left unchanged. */
+ * are device pointers. All data sets share each launch: a kNN search per data set, one symmetrize
+ * over the concatenated tables of each neighbour count, then init and SGD launches that walk every
+ * data set, so small data sets fill the GPU together. The work is captured once in a hipGraph and
+ * replayed by later calls with the same pointers, sizes and parameters (option "graph" 0: queued
+ * eagerly instead). n_streams is accepted for source compatibility and ignored. Runs on `stream`
+ * (where the inputs were produced) and returns when every fit is done. An error names the first
+ * failing data set; the handle's stage times are left unchanged. */
 int ncvis_fit_transform_many(ncvis_handle *h, size_t count, const float *const *dX, const size_t *N, const size_t *D,
                              float *const *dY, int n_streams, void *stream);
 

@@ -197,7 +197,8 @@ class Handle:
         self._hcheck(lib().ncvis_fit_transform_device(self._h, dX, N, D, dY, _stream(stream)))
 
     def fit_transform_many(self, jobs, n_streams: int = 4, stream=None) -> None:
-        """jobs: sequence of (dX, N, D, dY) device pointers and sizes, fitted n_streams at a time."""
+        """jobs: sequence of (dX, N, D, dY) device pointers and sizes, fitted together in shared
+        launches (replayed from a hipGraph for repeated calls); n_streams is ignored."""
         n = len(jobs)
         px = (vp * n)(*[j[0] for j in jobs])
         pn = (C.c_size_t * n)(*[j[1] for j in jobs])
