@@ -25,7 +25,7 @@ LIB = os.path.join(PKG, "libncvis_amd.so")
 
 # the kNN tile instantiations are split over several sources so they compile in parallel
 KNN_TILE_SOURCES = ["knn_q1_f32dma.hip", "knn_q1_f32.hip", "knn_q1_bf16.hip", "knn_q1_fp16.hip", "knn_t16.hip",
-                    "knn_wide.hip", "knn_wide_f32.hip"]
+                    "knn_wide.hip", "knn_wide_f32.hip", "knn_sym.hip"]
 HIP_SOURCES = KNN_TILE_SOURCES + ["knn.hip", "symmetrize.hip", "embed.hip", "abi.hip", "autotune.hip", "calibrate.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

@@ -627,6 +627,11 @@ int ncvis_tune(const char *key, double value) {
             if (value != 0 && value != 1 && value != 2 && value != 4)
                 throw std::invalid_argument("[ncvis_amd] knn_split: 0 (auto), 1, 2 or 4");
             t.knn_split = (int)value;
+        } else if (k == "knn_sym") {
+            if (value != -1 && value != 0 && !(value >= 1 && value <= 64))
+                throw std::invalid_argument("[ncvis_amd] knn_sym: -1 (by size), 0 (list tiles) or a bounding-row "
+                                            "stride 1..64");
+            t.knn_sym = (int)value;
         } else if (k == "knn_stage_mult") {
             if (value != 1 && value != 2) throw std::invalid_argument("[ncvis_amd] knn_stage_mult: 1 or 2");
             t.knn_stage_mult = (int)value;

@@ -21,6 +21,8 @@ struct Tuning {
     int knn_dma = -1;                // FP32 kNN tiles: stages by LDS DMA (1 or auto -1) or registers (0)
     int knn_stage_mult = 2;          // 16-bit kNN (k + 1 <= 16): LDS stage rows x1 or x2
     int knn_split = 0;               // candidate split of untuned searches (0: by workgroup count)
+    int knn_sym = -1;                // 16-bit kNN, k + 1 <= 16: symmetric threshold search with row
+                                     // bounds from every knn_sym-th row (0: list tiles; -1: by size)
 };
 Tuning &tuning();
 
@@ -32,7 +34,21 @@ struct TileVariant {
     int split = 1;
     int stage_mult = 2;
     int dma = 1;
+    int sym = 0;  // > 0: symmetric threshold search (knn_sym.hip), row bounds from every sym-th row
 };
+
+// Workspace of the symmetric threshold search (knn_sym.hip).
+struct SymBuffers {
+    float *T;       // [Npad] score bound of each row
+    int *cnt;       // [Npad + 1] bucket fill counts, then the fallback row count
+    uint2 *bucket;  // [N, sym_bucket_entries()] (score bits, label)
+    int *ovf;       // [N] rows left to the fallback
+};
+size_t sym_bucket_entries();
+// Candidate stage of the symmetric search with row bounds from every stride-th row: writes cand
+// [N, 32] (ids, -1 padded) for the rerank.
+void knn_sym_enqueue(int prec, int DK, const void *Xh, const float *cn, int64_t N, int64_t Npad, float beta,
+                     int K1, int stride, int split, const SymBuffers &w, int32_t *cand, hipStream_t st);
 
 // Exact k-nearest-neighbour table in the reference's findKNN row format: for each row the k+1
 // smallest (distance, label) pairs are found, the smallest is dropped and the remaining k are

@@ -486,7 +486,7 @@ struct KnnLayout {
     int DK, Dpad, QB, KL, lists, C, esz, nb;
     int64_t Npad;
     TileVariant tv;
-    size_t xp, part, mean, cs, xh, cn, cand, hd, hi, total;
+    size_t xp, part, mean, cs, xh, cn, cand, hd, hi, sT, scnt, sbucket, sovf, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -516,24 +516,36 @@ static KnnLayout knn_layout(int64_t N, int D, int k, int dist, int prec, const T
     L.lists = L.wide32 ? 4 : 2;  // candidate lists per query and candidate range
     if (L.tiled) {
         const int smax = max_split(L.Npad, L.lists, L.KL);
+        // symmetric threshold search (knn_sym.hip): 16-bit GEMMs, k + 1 <= 16, rows of at most
+        // 128 elements, at least two query blocks and 64 bounding rows
+        const bool sym_ok = !L.wide && L.KL == 16 && L.DK <= 4 && prec != PREC_FP32 && L.Npad >= 512;
         if (force) {
             L.tv = *force;
         } else if (!tuned_variant(prec, L.DK, L.KL, L.wide, N, L.tv)) {
-            // untuned: split the candidates until the launch has about 1024 workgroups
             const Tuning &tu = tuning();
             L.tv.stage_mult = tu.knn_stage_mult;
             L.tv.dma = tu.knn_dma != 0 ? 1 : 0;
-            const int64_t blocks = L.Npad / qtile;
+            L.tv.sym = (tu.knn_sym >= 0) ? tu.knn_sym : (N >= ((int64_t)1 << 18) ? 8 : 0);
+            if (!sym_ok) L.tv.sym = 0;
             L.tv.split = 1;
-            if (tu.knn_split > 0)
+            if (tu.knn_split > 0) {
                 L.tv.split = tu.knn_split;
-            else
+            } else if (L.tv.sym > 0) {
+                // the main pass pairs query blocks: split their sweeps until about 2048 workgroups
+                const int64_t pairs = (L.Npad / 256 + 1) / 2;
+                while (L.tv.split < 4 && pairs * L.tv.split < 2048) L.tv.split *= 2;
+            } else {
+                // list tiles: split the candidates until the launch has about 1024 workgroups
+                const int64_t blocks = L.Npad / qtile;
                 while (L.tv.split < smax && blocks * L.tv.split < 1024) L.tv.split *= 2;
+            }
         }
+        if (!sym_ok || L.tv.sym < 0 || (N + std::max(L.tv.sym, 1) - 1) / std::max(L.tv.sym, 1) < 64) L.tv.sym = 0;
         if (L.tv.split < 1) L.tv.split = 1;
-        if (L.tv.split > smax) L.tv.split = smax;
+        if (L.tv.split > (L.tv.sym > 0 ? 4 : smax)) L.tv.split = (L.tv.sym > 0) ? 4 : smax;
     }
-    L.C = L.lists * L.KL * L.tv.split;  // candidates per query for the rerank: 32, 64 or 128
+    // candidates per query for the rerank: 32, 64 or 128 (the symmetric search writes 32)
+    L.C = (L.tv.sym > 0) ? 32 : L.lists * L.KL * L.tv.split;
     L.esz = (prec == PREC_FP32) ? 4 : 2;
     L.nb = (int)std::min<int64_t>(512, std::max<int64_t>(1, cdiv(N, 256)));
     size_t o = 0;
@@ -550,6 +562,12 @@ static KnnLayout knn_layout(int64_t N, int D, int k, int dist, int prec, const T
         L.xh = take((size_t)L.Npad * L.Dpad * L.esz);
         L.cn = take(sizeof(float) * L.Npad);
         L.cand = take(sizeof(int32_t) * N * L.C);
+        if (L.tv.sym > 0) {
+            L.sT = take(sizeof(float) * L.Npad);
+            L.scnt = take(sizeof(int) * (L.Npad + 1));
+            L.sbucket = take(sizeof(uint2) * N * sym_bucket_entries());
+            L.sovf = take(sizeof(int) * N);
+        }
     } else {
         L.hd = take(sizeof(float) * N * (k + 1));
         L.hi = take(sizeof(int) * N * (k + 1));
@@ -632,6 +650,11 @@ void knn_enqueue(const float *dX, int64_t N, int D, int k, int dist, int prec, i
             knn_tiles_wide_f32(KL, DK, N, Npad, xh, cn, beta, cand, st, S);
         else if (L.wide)
             knn_tiles_wide(prec, KL, DK, N, Npad, xh, cn, beta, cand, st, S);
+        else if (L.tv.sym > 0)
+            knn_sym_enqueue(prec, DK, xh, cn, N, Npad, beta, k + 1, L.tv.sym, S,
+                            SymBuffers{(float *)(w + L.sT), (int *)(w + L.scnt), (uint2 *)(w + L.sbucket),
+                                       (int *)(w + L.sovf)},
+                            cand, st);
         else if (KL == 16)
             knn_tiles16(prec, DK, N, Npad, xh, cn, beta, cand, st, L.tv);
         else if (prec == PREC_FP32 && L.tv.dma != 0)
