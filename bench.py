@@ -324,6 +324,18 @@ def main():
                 "peak": MFMA_PEAK_TFLOPS[args.precision], "unit": "TFLOP/s",
                 "traffic": traffic.get("k_knn_tiles"), "launch_ms": stage["knn_tiles"]}
     roof_knn["frac"] = roof_knn["achieved"] / roof_knn["peak"]
+    # the MFMA work the chosen search variant executes: the symmetric search scores each pair of
+    # 256-row blocks once (diagonal blocks in full) plus its bounding pass over every sym-th row
+    plan = abi.knn_plan(N, D, 15, "euclidean", args.precision)
+    if plan["sym"] > 0:
+        nbI = -(-N // 256)
+        executed = 2.0 * D * 256 * 256 * nbI * (nbI + 1) / 2 + 2.0 * D * N * (-(-N // plan["sym"]))
+        roof_knn["kernel"] = "k_sym_bound + k_sym_main + k_sym_select"
+    else:
+        executed = knn_flops
+    roof_knn["variant"] = plan
+    roof_knn["executed_flops"] = executed
+    roof_knn["executed_frac"] = executed / knn_tiles_s / 1e12 / roof_knn["peak"]
     roof_sgd = {"bound": "hbm", "kernel": "k_sgd_epoch", "achieved": sgd_bytes / sgd_s / 1e9,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": traffic.get("k_sgd_epoch"),
                 "launch_ms": stage["sgd_kernel"] / n_epochs}

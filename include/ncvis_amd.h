@@ -73,7 +73,9 @@ int ncvis_set_option(ncvis_handle *h, const char *key, double value);
  * min(E / 16, "min_chains"), at most "max_chains"),
  * "knn_dma" (FP32 tiles: -1 auto = 1 LDS DMA, 0 register-staged), "knn_stage_mult" (16-bit
  * tiles: 1 or 2 x the LDS stage rows), "knn_split" (candidate ranges per query block: 0 auto,
- * else 1, 2 or 4; each range keeps lists of its own, so the table is unchanged). */
+ * else 1, 2 or 4; each range keeps lists of its own, so the table is unchanged), "knn_sym"
+ * (16-bit searches with k + 1 <= 16 and D <= 128: -1 by size (2^18 rows or more), 0 the list
+ * tiles, s in 1..64 the symmetric threshold search with row bounds from every s-th row). */
 int ncvis_tune(const char *key, double value);
 
 /* ncvis::NCVis::fit_transform: X [N, D] and Y [N, d] are host, row-major float32. */
@@ -128,7 +130,9 @@ int ncvis_optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const in
 
 /* ---- kNN tile autotuning (no reference counterpart: the reference's HNSW has no tiles) ----
  * ncvis_autotune_knn times every admissible launch variant of the kNN candidate tiles for one
- * search shape (candidate split 1/2/4, the 16-bit LDS stage size, FP32 LDS DMA) on dX [N, D]
+ * search shape (candidate split 1/2/4, the 16-bit LDS stage size, FP32 LDS DMA, and for 16-bit
+ * searches of k + 1 <= 16 and D <= 128 the symmetric threshold search with row bounds from every
+ * 4th, 8th or 16th row) on dX [N, D]
  * (device rows; NULL: uniform synthetic rows) and keeps the fastest in a process-wide table keyed
  * by device architecture, GEMM precision, padded width, list length and floor(log2 N). Every
  * variant gives the exact table on exactly scored data; more splits only add candidates to the
@@ -138,9 +142,15 @@ int ncvis_optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const in
  * search of a process; ncvis_load_tune_cache replaces it from a file (NULL or "": empties it).
  * Errors: ncvis_autotune_last_error. */
 int ncvis_autotune_knn(const float *dX, size_t N, size_t D, size_t k, int dist, int precision, const char *cache_path,
-                       int *split_out, int *stage_mult_out, int *dma_out, double *ms_out);
+                       int *split_out, int *stage_mult_out, int *dma_out, int *sym_out, double *ms_out);
 int ncvis_load_tune_cache(const char *path);
 int ncvis_tune_cache_size(void);
+
+/* The kNN launch variant a search of this shape uses now (tuning table, knobs, size rules): the
+ * candidate split, the 16-bit stage multiplier, FP32 LDS DMA and the symmetric search's
+ * bounding-row stride (0: list tiles). No reference counterpart. */
+int ncvis_knn_plan(size_t N, size_t D, size_t k, int dist, int precision, int *split_out, int *stage_mult_out,
+                   int *dma_out, int *sym_out);
 const char *ncvis_autotune_last_error(void);
 
 /* ---- measurement (no reference counterpart) ----

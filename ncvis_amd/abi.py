@@ -18,6 +18,7 @@ EXPORTS = (
     "ncvis_symmetrize_device", "ncvis_init_embedding_device", "ncvis_optimize_device",
     "ncvis_nce_loss_device", "ncvis_tune", "ncvis_fit_transform_many", "ncvis_optimize_device_noalloc",
     "ncvis_autotune_knn", "ncvis_load_tune_cache", "ncvis_tune_cache_size", "ncvis_autotune_last_error",
+    "ncvis_knn_plan",
     "ncvis_calibrate_rmw", "ncvis_calibrate_last_error",
 )
 
@@ -65,7 +66,9 @@ def lib():
         L.ncvis_tune.argtypes = [C.c_char_p, C.c_double]
         L.ncvis_autotune_knn.argtypes = [vp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int, C.c_int, C.c_char_p,
                                          C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                         C.POINTER(C.c_double)]
+                                         C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.ncvis_knn_plan.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.ncvis_load_tune_cache.argtypes = [C.c_char_p]
         L.ncvis_calibrate_rmw.argtypes = [C.c_size_t, C.c_int64, C.c_int, C.POINTER(C.c_double),
                                           C.POINTER(C.c_double)]
@@ -98,14 +101,24 @@ def autotune_knn(dX, N: int, D: int, k: int, distance: str = "euclidean", precis
                  cache_path=None):
     """Times the kNN tile variants for this search shape (dX: device pointer, or None for synthetic
     rows) and keeps the fastest; cache_path: None (memory only), "" ($NCVIS_AMD_TUNE_CACHE) or a
-    JSON file. Returns {"split", "stage_mult", "dma", "ms"}."""
-    sp, sm, dm, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    JSON file. Returns {"split", "stage_mult", "dma", "sym", "ms"}."""
+    sp, sm, dm, sy, ms = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
     path = None if cache_path is None else str(cache_path).encode()
     rc = lib().ncvis_autotune_knn(dX or None, N, D, k, DIST[distance], PREC[precision], path, C.byref(sp),
-                                  C.byref(sm), C.byref(dm), C.byref(ms))
+                                  C.byref(sm), C.byref(dm), C.byref(sy), C.byref(ms))
     if rc != 0:
         raise RuntimeError(lib().ncvis_autotune_last_error().decode())
-    return {"split": sp.value, "stage_mult": sm.value, "dma": dm.value, "ms": ms.value}
+    return {"split": sp.value, "stage_mult": sm.value, "dma": dm.value, "sym": sy.value, "ms": ms.value}
+
+
+def knn_plan(N: int, D: int, k: int, distance: str = "euclidean", precision: str = "fp16"):
+    """The kNN launch variant a search of this shape uses now: {"split", "stage_mult", "dma", "sym"}
+    (sym: the symmetric search's bounding-row stride, 0 for the list tiles)."""
+    sp, sm, dm, sy = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    if lib().ncvis_knn_plan(N, D, k, DIST[distance], PREC[precision], C.byref(sp), C.byref(sm), C.byref(dm),
+                            C.byref(sy)) != 0:
+        raise RuntimeError(lib().ncvis_autotune_last_error().decode())
+    return {"split": sp.value, "stage_mult": sm.value, "dma": dm.value, "sym": sy.value}
 
 
 def calibrate_rmw(rows: int, threads: int, iters: int = 512):

@@ -1,6 +1,7 @@
 // Autotuner of the kNN candidate tiles and its on-disk cache.
 //
-// The launch choices a TileVariant holds (candidate split, 16-bit LDS stage multiplier, FP32 LDS
+// The launch choices a TileVariant holds (symmetric threshold search and its bounding-row stride,
+// candidate split, 16-bit LDS stage multiplier, FP32 LDS
 // DMA) never change which candidates a list keeps from its range, so every variant yields the
 // exact table on exactly scored data, and more splits only add candidates to the exact rerank.
 // Their speed, though, depends on the search's shape: a small data set needs the candidate
@@ -150,6 +151,7 @@ void load_file_locked(TuneTable &t, const std::string &path) {
         e.tv.split = o.i("split", 1);
         e.tv.stage_mult = o.i("stage_mult", 2);
         e.tv.dma = o.i("dma", 1);
+        e.tv.sym = o.i("sym", 0);
         e.ms = o.f("ms");
         if (key.prec < 0 || key.dk < 0 || key.kl < 0 || key.nlog2 < 0) continue;
         t.entries[key] = e;
@@ -169,9 +171,9 @@ void save_file_locked(const TuneTable &t, const std::string &path) {
             char buf[512];
             std::snprintf(buf, sizeof(buf),
                           "%s\n    {\"arch\": \"%s\", \"prec\": %d, \"dk\": %d, \"kl\": %d, \"wide\": %d, \"nlog2\": %d, "
-                          "\"split\": %d, \"stage_mult\": %d, \"dma\": %d, \"ms\": %.5f}",
+                          "\"split\": %d, \"stage_mult\": %d, \"dma\": %d, \"sym\": %d, \"ms\": %.5f}",
                           first ? "" : ",", k.arch.c_str(), k.prec, k.dk, k.kl, k.wide, k.nlog2, e.tv.split,
-                          e.tv.stage_mult, e.tv.dma, e.ms);
+                          e.tv.stage_mult, e.tv.dma, e.tv.sym, e.ms);
             f << buf;
             first = false;
         }
@@ -227,7 +229,7 @@ extern "C" {
 const char *ncvis_autotune_last_error(void) { return g_tune_error.c_str(); }
 
 int ncvis_autotune_knn(const float *dX, size_t N, size_t D, size_t k, int dist, int precision, const char *cache_path,
-                       int *split_out, int *stage_mult_out, int *dma_out, double *ms_out) {
+                       int *split_out, int *stage_mult_out, int *dma_out, int *sym_out, double *ms_out) {
     try {
         if (N < 2 || D < 1 || k < 1 || k > N - 1) throw std::invalid_argument("[ncvis_amd] autotune: need 1 <= k <= N-1");
         if (dist < DIST_L2 || dist > DIST_CORRELATION) throw std::invalid_argument("[ncvis_amd] autotune: bad distance");
@@ -253,6 +255,7 @@ int ncvis_autotune_knn(const float *dX, size_t N, size_t D, size_t k, int dist, 
         for (int s = 1; s <= 4; s *= 2) {
             TileVariant tv = base;
             tv.split = s;
+            tv.sym = 0;
             std::vector<int> sms{base.stage_mult}, dmas{base.dma};
             if (precision != PREC_FP32 && !wide && k + 1 <= 16) sms = {1, 2};
             if (precision == PREC_FP32 && !wide) dmas = {0, 1};
@@ -260,6 +263,17 @@ int ncvis_autotune_knn(const float *dX, size_t N, size_t D, size_t k, int dist, 
                 for (int dm : dmas) {
                     tv.stage_mult = sm;
                     tv.dma = dm;
+                    cands.push_back(tv);
+                }
+        }
+        // the symmetric threshold search (16-bit GEMMs, k + 1 <= 16, rows of at most 128 elements):
+        // bounding-row strides times its own main-pass splits; small searches keep the list tiles
+        if (precision != PREC_FP32 && k + 1 <= 16 && D <= 128 && N >= ((size_t)1 << 16)) {
+            for (int stride : {4, 8, 16})
+                for (int s = 1; s <= 4; s *= 2) {
+                    TileVariant tv = base;
+                    tv.sym = stride;
+                    tv.split = s;
                     cands.push_back(tv);
                 }
         }
@@ -288,7 +302,7 @@ int ncvis_autotune_knn(const float *dX, size_t N, size_t D, size_t k, int dist, 
         for (const TileVariant &cand : cands) {
             // the layout clamps the split to what the search admits: time each effective variant once
             const TileVariant tv = knn_effective_variant((int64_t)N, (int)D, (int)k, dist, precision, cand);
-            const int code = tv.split * 100 + tv.stage_mult * 10 + tv.dma;
+            const int code = tv.sym * 1000 + tv.split * 100 + tv.stage_mult * 10 + tv.dma;
             if (std::find(seen.begin(), seen.end(), code) != seen.end()) continue;
             seen.push_back(code);
             double ms = 1e300;
@@ -324,6 +338,7 @@ int ncvis_autotune_knn(const float *dX, size_t N, size_t D, size_t k, int dist, 
         if (split_out) *split_out = best.split;
         if (stage_mult_out) *stage_mult_out = best.stage_mult;
         if (dma_out) *dma_out = best.dma;
+        if (sym_out) *sym_out = best.sym;
         if (ms_out) *ms_out = best_ms;
         g_tune_error.clear();
         return NCVIS_OK;
@@ -361,6 +376,26 @@ int ncvis_tune_cache_size(void) {
     TuneTable &t = table();
     std::lock_guard<std::mutex> lk(t.mu);
     return (int)t.entries.size();
+}
+
+int ncvis_knn_plan(size_t N, size_t D, size_t k, int dist, int precision, int *split_out, int *stage_mult_out,
+                   int *dma_out, int *sym_out) {
+    try {
+        if (N < 2 || D < 1 || k < 1 || k > N - 1) throw std::invalid_argument("[ncvis_amd] knn plan: need 1 <= k <= N-1");
+        if (dist < DIST_L2 || dist > DIST_CORRELATION) throw std::invalid_argument("[ncvis_amd] knn plan: bad distance");
+        if (precision < PREC_FP32 || precision > PREC_FP16)
+            throw std::invalid_argument("[ncvis_amd] knn plan: bad precision");
+        const TileVariant tv = knn_variant((int64_t)N, (int)D, (int)k, dist, precision);
+        if (split_out) *split_out = tv.split;
+        if (stage_mult_out) *stage_mult_out = tv.stage_mult;
+        if (dma_out) *dma_out = tv.dma;
+        if (sym_out) *sym_out = tv.sym;
+        g_tune_error.clear();
+        return NCVIS_OK;
+    } catch (const std::exception &e) {
+        g_tune_error = e.what();
+        return NCVIS_EINVAL;
+    }
 }
 
 }  // extern "C"

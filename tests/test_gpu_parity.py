@@ -471,3 +471,31 @@ def test_knn_autotune_cache_round_trip(torch_cuda, tmp_path):
         assert abi.load_tune_cache(path) == 2
     finally:
         abi.load_tune_cache(None)
+
+
+def test_knn_autotune_times_the_symmetric_search(torch_cuda, tmp_path):
+    # from 2^16 rows the autotuner also times the symmetric threshold search; whichever variant
+    # it keeps, the plan reports it and the table equals the list tiles' on integer data
+    from ncvis_amd import abi
+    import json
+    torch = torch_cuda
+    _say("knn autotune symmetric")
+    N = 70_000
+    X, _ = synthetic.digits_like(N=N, D=64, seed=41)
+    dX = _dev(torch, X)
+    path = tmp_path / "tiles.json"
+    try:
+        best = abi.autotune_knn(dX.data_ptr(), N, 64, 15, "euclidean", "fp16", cache_path=path)
+        assert best["sym"] in (0, 4, 8, 16) and best["ms"] > 0
+        assert json.loads(path.read_text())["entries"][0]["sym"] == best["sym"]
+        plan = abi.knn_plan(N, 64, 15, "euclidean", "fp16")
+        assert plan["sym"] == best["sym"] and plan["split"] == best["split"]
+        gi, gd = gpu_knn(torch, X, 15, "euclidean", "fp16")
+    finally:
+        abi.load_tune_cache(None)
+    try:
+        abi.tune("knn_sym", 0)
+        li, ld = gpu_knn(torch, X, 15, "euclidean", "fp16")
+    finally:
+        abi.tune("knn_sym", -1)
+    assert np.array_equal(gi, li) and np.array_equal(gd, ld)
