@@ -333,8 +333,10 @@ __global__ __launch_bounds__(256, 2) void k_sym_main(const void *__restrict__ Xh
     const int64_t Lt = len1 + len2;
     const int64_t vb = Lt * blockIdx.y / gridDim.y, ve = Lt * (blockIdx.y + 1) / gridDim.y;
     if (vb >= ve) return;  // uniform over the workgroup
-    // virtual stage v: stage I1 * SPB + v of block I1, then stage I2 * SPB + (v - len1) of block I2
-#define NCVIS_SYM_STAGE(V) ((V) < len1 ? I1 * SPB + (V) : I2 * SPB + ((V)-len1))
+    // virtual stage v: block I1's stages from the last one down to its own rows, then block I2's:
+    // every workgroup sweeps the candidate stages in the same (descending) order, so workgroups
+    // running together read the same stages and share them in L2
+#define NCVIS_SYM_STAGE(V) ((V) < len1 ? NS - 1 - (V) : NS - 1 - ((V)-len1))
 
     uint4 qf[2][KS];
     int qi[2] = {0, 0};
