@@ -190,15 +190,27 @@ __global__ __launch_bounds__(256) void k_colstat_seg(const FitSeg *__restrict__ 
     }
 }
 
-// One block per data set. MODE 0: mean[c] = sum / N; MODE 1: sigma[c] = sqrtf(sum / N).
+// One 256-thread block per data set. MODE 0: mean[c] = sum / N; MODE 1: sigma[c] = sqrtf(sum / N).
+// The block partials are summed in a fixed order (thread t takes partials t, t + 256, ..., then a
+// fixed tree), so the statistics do not depend on scheduling.
 template <int MODE>
-__global__ void k_finalize_seg(const FitSeg *__restrict__ s, int d) {
+__global__ __launch_bounds__(256) void k_finalize_seg(const FitSeg *__restrict__ s, int d) {
+    __shared__ double red[256];
     const FitSeg &g = s[blockIdx.x];
-    for (int c = threadIdx.x; c < d; c += blockDim.x) {
+    for (int c = 0; c < d; ++c) {
         double acc = 0.0;
-        for (int b = 0; b < g.nb; ++b) acc += g.part[(int64_t)b * d + c];
-        const float v = (float)(acc / (double)g.N);
-        g.stats[MODE * d + c] = (MODE == 0) ? v : sqrtf(v);
+        for (int b = threadIdx.x; b < g.nb; b += 256) acc += g.part[(int64_t)b * d + c];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const float v = (float)(red[0] / (double)g.N);
+            g.stats[MODE * d + c] = (MODE == 0) ? v : sqrtf(v);
+        }
+        __syncthreads();
     }
 }
 

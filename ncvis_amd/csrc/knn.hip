@@ -517,8 +517,10 @@ static KnnLayout knn_layout(int64_t N, int D, int k, int dist, int prec, const T
     if (L.tiled) {
         const int smax = max_split(L.Npad, L.lists, L.KL);
         // symmetric threshold search (knn_sym.hip): 16-bit GEMMs, k + 1 <= 16, rows of at most
-        // 128 elements, at least two query blocks and 64 bounding rows
-        const bool sym_ok = !L.wide && L.KL == 16 && L.DK <= 4 && prec != PREC_FP32 && L.Npad >= 512;
+        // 128 elements, at least two query blocks and 64 bounding rows, and at most 2^23 rows (its
+        // buckets take 4 KiB per row)
+        const bool sym_ok = !L.wide && L.KL == 16 && L.DK <= 4 && prec != PREC_FP32 && L.Npad >= 512 &&
+                            N <= ((int64_t)1 << 23);
         if (force) {
             L.tv = *force;
         } else if (!tuned_variant(prec, L.DK, L.KL, L.wide, N, L.tv)) {
