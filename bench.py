@@ -209,6 +209,9 @@ def main():
     ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle legs (parity, CPU baseline)")
     ap.add_argument("--no-companions", action="store_true", help="skip the other-precision fits")
+    ap.add_argument("--no-hnsw", action="store_true", help="skip the CPU HNSW leg (the parity leg still runs)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="process-wide tuning knob (ncvis_tune) set before the fits; repeatable")
     ap.add_argument("--no-autotune", action="store_true", help="batch64: keep the default tile variants")
     ap.add_argument("--fit-only", action="store_true",
                     help="time the fits and print the stage times only (short runs under a profiler)")
@@ -222,6 +225,11 @@ def main():
     from ncvis_amd import abi
     from ncvis_amd._ncvis import find_ab_params, negative_plan_for
 
+    tuned = {}
+    for kv in args.tune:
+        key, _, val = kv.partition("=")
+        abi.tune(key, float(val))
+        tuned[key] = float(val)
     if args.workload == "batch64":
         run_batch(args)
         return
@@ -326,29 +334,37 @@ def main():
     roof_knn["frac"] = roof_knn["achieved"] / roof_knn["peak"]
     # the MFMA work the chosen search variant executes: the symmetric search scores each pair of
     # 256-row blocks once (diagonal blocks in full) plus its bounding pass over every sym-th row
-    plan = abi.knn_plan(N, D, 15, "euclidean", args.precision)
-    if plan["sym"] > 0:
+    kplan = abi.knn_plan(N, D, k, w["distance"], args.precision)
+    if kplan["sym"] > 0:
         nbI = -(-N // 256)
-        executed = 2.0 * D * 256 * 256 * nbI * (nbI + 1) / 2 + 2.0 * D * N * (-(-N // plan["sym"]))
+        executed = 2.0 * D * 256 * 256 * nbI * (nbI + 1) / 2 + 2.0 * D * N * (-(-N // kplan["sym"]))
         roof_knn["kernel"] = "k_sym_bound + k_sym_main + k_sym_select"
     else:
         executed = knn_flops
-    roof_knn["variant"] = plan
+    roof_knn["variant"] = kplan
     roof_knn["executed_flops"] = executed
     roof_knn["executed_frac"] = executed / knn_tiles_s / 1e12 / roof_knn["peak"]
     roof_sgd = {"bound": "hbm", "kernel": "k_sgd_epoch", "achieved": sgd_bytes / sgd_s / 1e9,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": traffic.get("k_sgd_epoch"),
-                "launch_ms": stage["sgd_kernel"] / n_epochs}
+                "launch_ms": stage["sgd_kernel"] / s_last["sgd_launches"], "launches": s_last["sgd_launches"]}
     roof_sgd["frac"] = roof_sgd["achieved"] / roof_sgd["peak"]
-    # the epoch's real bound: one random 8-byte-row load + device-scope 64-bit CAS per sample,
-    # calibrated on a table of Y's size with the epoch's thread count
+    # the epoch's real bound: one random 8-byte-row read + update per sample, calibrated on a
+    # table of Y's size with the epoch's thread count -- a device-scope 64-bit CAS (mode 2) or a
+    # re-read + plain store of the XCD's own copy (mode 3)
     if s_last["chains"] >= 256:
-        cal = abi.calibrate_rmw(N, (s_last["chains"] // 256) * 256, 512)
+        thr = (s_last["chains"] // 256) * 256
+        cal = abi.calibrate_rmw(N, thr, 512)
+        copies = abi.calibrate_rmw_copies(N, thr, 512)
         samples_per_s = n_samples / sgd_s
+        ceiling = copies if s_last["sgd_mode"] == 3 else cal["load_cas_rows_per_s"]
         roof_sgd["rmw_ceiling"] = {"samples_per_s": samples_per_s,
+                                   "pattern": "per-XCD copy re-read + store" if s_last["sgd_mode"] == 3
+                                   else "device-scope 64-bit CAS",
+                                   "ceiling_rows_per_s": ceiling,
+                                   "copy_rows_per_s": copies,
                                    "load_cas_rows_per_s": cal["load_cas_rows_per_s"],
                                    "load_rows_per_s": cal["load_rows_per_s"],
-                                   "frac": samples_per_s / cal["load_cas_rows_per_s"]}
+                                   "frac": samples_per_s / ceiling}
     dominant = roof_knn if knn_tiles_s >= sgd_s else roof_sgd
 
     # ---- the host-buffer entry point (X and Y cross PCIe): reported beside, never as `value`
@@ -422,6 +438,7 @@ def main():
                   "pass": bool(abs(pg - po) <= 0.01 and dL <= 0.02)}
         log(f"parity: preservation {pg:.4f} vs {po:.4f}; loss {losses[0]:.4g} vs {losses[1]:.4g}")
 
+    if not args.no_cpu_baseline and not args.no_hnsw:
         log("oracle HNSW kNN + symmetrize")
         Xc = X if w["distance"] == "euclidean" else oracle.preprocess(X, w["distance"])
         t0 = time.perf_counter()
@@ -459,7 +476,8 @@ def main():
         "config": {"workload": w["desc"], "N": N, "D": D, "k": k, "d": d, "n_epochs": n_epochs,
                    "n_init_epochs": n_init, "noise_plan": "%s (sum 1+n = %d)" % (
                        "default" if w["n_noise"] is None else f"n_noise={w['n_noise']}", int(np.sum(1 + plan))),
-                   "knn_precision": args.precision, "chains": s_last["chains"], "distance": w["distance"]},
+                   "knn_precision": args.precision, "chains": s_last["chains"], "distance": w["distance"],
+                   "sgd_mode": s_last.get("sgd_mode"), "tuning": tuned or None},
         "recall@15": knn_quality[args.precision]["recall@15"],
         "knn_vs_fp64": knn_quality,
         "companions": companions,

@@ -59,8 +59,13 @@ const char *ncvis_last_error(const ncvis_handle *h);
 
 /* Options (no reference counterpart):
  *   "knn_precision"   NCVIS_PREC_*  (default NCVIS_PREC_FP16)
- *   "atomic_updates"  1: Hogwild Y updates with atomics (default when more than one chain runs),
- *                     0: plain read-modify-write stores like the reference
+ *   "atomic_updates"  -1 (default): one chain with plain stores; many chains on per-XCD copies
+ *                     of Y (d <= 4 and tuning "sgd_replicas" 1, else device atomics);
+ *                     0: plain read-modify-write stores like the reference; 1: device-scope
+ *                     atomics on Y (no update lost); 2: per-XCD copies of Y (d <= 4): each XCD
+ *                     updates its own copy with plain stores in its L2 (a chain of the same XCD
+ *                     can overwrite a concurrent update of a row, the reference's Hogwild race),
+ *                     and the copies are merged after every one of "sgd_sync" sub-epochs
  *   "exact_threads"   1: run exactly n_threads chains (default 0: GPU-sized chain count)
  *   "chains"          override the SGD chain count (0 = none)
  *   "graph"           1: ncvis_fit_transform_many captures its work in a hipGraph and replays it
@@ -69,7 +74,10 @@ int ncvis_set_option(ncvis_handle *h, const char *key, double value);
 
 /* Process-wide tuning knobs (no reference counterpart). None changes a single-chain fit; the kNN
  * knobs never change the kNN table. Keys: "sgd_prefetch" (1,2,4,8), "sgd_plain_far" (0/1),
- * "sgd_fast_math" (0/1), "edges_per_chain" (auto chain count = E / this, at least
+ * "sgd_fast_math" (0/1), "sgd_replicas" (1: many-chain epochs on per-XCD copies of Y when the
+ * atomics are automatic, 0: device atomics), "sgd_sync" (sub-epochs per epoch, 1..64: every chain
+ * writes its source row out after each, and the copies are merged), "sgd_edge_flush" (0/1: with
+ * copies, the source row is written after every edge), "edges_per_chain" (auto chain count = E / this, at least
  * min(E / 16, "min_chains"), at most "max_chains"),
  * "knn_dma" (FP32 tiles: -1 auto = 1 LDS DMA, 0 register-staged), "knn_stage_mult" (16-bit
  * tiles: 1 or 2 x the LDS stage rows), "knn_split" (candidate ranges per query block: 0 auto,
@@ -98,9 +106,16 @@ int ncvis_fit_transform_many(ncvis_handle *h, size_t count, const float *const *
                              float *const *dY, int n_streams, void *stream);
 
 /* Stage times of the last fit (milliseconds): [knn, symmetrize, init, optimize, total, knn_tiles,
- * knn_rerank, sgd_kernel] (sgd_kernel: the epoch kernels alone, summed over epochs), the edge
+ * knn_rerank, sgd_kernel] (sgd_kernel: the epoch kernel launches alone, summed), the edge
  * count E, the final Q and the effective chain count. */
 int ncvis_get_stats(const ncvis_handle *h, double *stage_ms8, int64_t *n_edges, float *final_Q, int64_t *chains);
+
+/* SGD epoch mode of the last fit (no reference counterpart): 0 one exact chain with plain
+ * stores (the reference at one thread), 1 exact chains with float atomics, 2 many chains with
+ * device-scope atomics on Y, 3 many chains on per-XCD copies of Y merged after every sub-epoch;
+ * launches: epoch kernel launches (epochs x sub-epochs), the count the sgd_kernel time of
+ * ncvis_get_stats sums over. Either output may be NULL. */
+int ncvis_get_sgd_mode(const ncvis_handle *h, int *mode, int *launches);
 
 /* ---- stage entry points (device pointers) ---- */
 const char *ncvis_get_last_error(void);
@@ -121,6 +136,8 @@ int ncvis_init_embedding_device(size_t N, size_t d, const int64_t *d_row_ptr, co
 
 /* SGD epochs. Q is host in/out; q_trace (host, optional) receives Q after each epoch.
  * n_threads = the exact chain count (< 1: GPU-sized). n_noise entries must be below 2^32 - 1.
+ * atomic_updates: 0 plain stores, 1 or -1 device atomics, 2 per-XCD copies of Y (d <= 4; the
+ * copies are allocated here). 
  * The graph is validated first (row_ptr [N+1] from 0 to n_edges, never decreasing; columns in
  * [0, N)); a bad graph returns NCVIS_EINVAL. dY needs only float alignment. */
 int ncvis_optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const int32_t *d_col, int64_t n_edges,
@@ -154,18 +171,22 @@ int ncvis_knn_plan(size_t N, size_t D, size_t k, int dist, int precision, int *s
 const char *ncvis_autotune_last_error(void);
 
 /* ---- measurement (no reference counterpart) ----
- * The many-chain SGD epoch is bound by random 8-byte-row loads each followed by a device-scope
- * 64-bit compare-and-swap. ncvis_calibrate_rmw measures that pattern's own ceiling: `threads`
- * threads (multiple of 256) each load `iters` pcg64-random rows of a table of `rows` 8-byte
- * entries, once read-only and once with a compare-and-swap update; outputs rows per second. */
+ * The many-chain SGD epochs are bound by random 8-byte-row loads each followed by an update of
+ * that row. ncvis_calibrate_rmw measures the device-atomic epoch's pattern: `threads` threads
+ * (multiple of 256) each load `iters` pcg64-random rows of a table of `rows` 8-byte entries, once
+ * read-only and once with a device-scope 64-bit compare-and-swap update; outputs rows per
+ * second. ncvis_calibrate_rmw_copies measures the per-XCD-copy epoch's pattern: the table is
+ * held once per XCD, each wave reads a random row of its XCD's copy, reads it again and stores
+ * the update (workgroup scope, served by that XCD's L2). */
 int ncvis_calibrate_rmw(size_t rows, int64_t threads, int iters, double *load_rows_per_s, double *cas_rows_per_s);
+int ncvis_calibrate_rmw_copies(size_t rows, int64_t threads, int iters, double *rows_per_s);
 const char *ncvis_calibrate_last_error(void);
 
 /* ncvis_optimize_device without any device allocation. No counterpart in the reference: it is
  * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc). Same arguments and results as ncvis_optimize_device;
  * its scratch (chain states, Q copies, the per-epoch Q buffer, the graph-check flag) lives in one
- * static device array per GPU (3.75 MiB, held until the process ends). Limits: at most 2^17
+ * static device array per GPU (4.75 MiB, held until the process ends). Limits: at most 2^17
  * chains (n_threads < 1 picks the GPU-sized count, which stays within it by default) and at
  * most 2^16 epochs; larger requests return NCVIS_EINVAL before any GPU work. Every call on a GPU
  * shares that GPU's array, so calls that overlap on one GPU (other host threads, other streams)

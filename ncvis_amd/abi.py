@@ -14,12 +14,12 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libncvis_am
 # every symbol include/ncvis_amd.h declares
 EXPORTS = (
     "ncvis_create", "ncvis_destroy", "ncvis_last_error", "ncvis_set_option", "ncvis_fit_transform",
-    "ncvis_fit_transform_device", "ncvis_get_stats", "ncvis_get_last_error", "ncvis_knn_device",
+    "ncvis_fit_transform_device", "ncvis_get_stats", "ncvis_get_sgd_mode", "ncvis_get_last_error", "ncvis_knn_device",
     "ncvis_symmetrize_device", "ncvis_init_embedding_device", "ncvis_optimize_device",
     "ncvis_nce_loss_device", "ncvis_tune", "ncvis_fit_transform_many", "ncvis_optimize_device_noalloc",
     "ncvis_autotune_knn", "ncvis_load_tune_cache", "ncvis_tune_cache_size", "ncvis_autotune_last_error",
     "ncvis_knn_plan",
-    "ncvis_calibrate_rmw", "ncvis_calibrate_last_error",
+    "ncvis_calibrate_rmw", "ncvis_calibrate_rmw_copies", "ncvis_calibrate_last_error",
 )
 
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
@@ -52,6 +52,7 @@ def lib():
                                                C.POINTER(C.c_size_t), C.POINTER(vp), C.c_int, vp]
         L.ncvis_get_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                       C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+        L.ncvis_get_sgd_mode.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.ncvis_knn_device.argtypes = [vp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int, C.c_int, vp,
                                        vp, vp]
         L.ncvis_symmetrize_device.argtypes = [vp, vp, C.c_size_t, C.c_size_t, vp, vp, vp,
@@ -72,6 +73,7 @@ def lib():
         L.ncvis_load_tune_cache.argtypes = [C.c_char_p]
         L.ncvis_calibrate_rmw.argtypes = [C.c_size_t, C.c_int64, C.c_int, C.POINTER(C.c_double),
                                           C.POINTER(C.c_double)]
+        L.ncvis_calibrate_rmw_copies.argtypes = [C.c_size_t, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         L.ncvis_calibrate_last_error.argtypes = []
         L.ncvis_calibrate_last_error.restype = C.c_char_p
         L.ncvis_tune_cache_size.argtypes = []
@@ -127,6 +129,15 @@ def calibrate_rmw(rows: int, threads: int, iters: int = 512):
     if lib().ncvis_calibrate_rmw(rows, threads, iters, C.byref(lo), C.byref(ca)) != 0:
         raise RuntimeError(lib().ncvis_calibrate_last_error().decode())
     return {"load_rows_per_s": lo.value, "load_cas_rows_per_s": ca.value}
+
+
+def calibrate_rmw_copies(rows: int, threads: int, iters: int = 512) -> float:
+    """Rate (rows/s) of the per-XCD-copy update: random row read, read again, plain store, on the
+    XCD's own copy of a table of `rows` 8-byte entries."""
+    r = C.c_double()
+    if lib().ncvis_calibrate_rmw_copies(rows, threads, iters, C.byref(r)) != 0:
+        raise RuntimeError(lib().ncvis_calibrate_last_error().decode())
+    return r.value
 
 
 def load_tune_cache(path=None) -> int:
@@ -228,10 +239,12 @@ class Handle:
         ms = (C.c_double * 8)()
         e, c, q = C.c_int64(), C.c_int64(), C.c_float()
         lib().ncvis_get_stats(self._h, ms, C.byref(e), C.byref(q), C.byref(c))
+        mode, launches = C.c_int(), C.c_int()
+        lib().ncvis_get_sgd_mode(self._h, C.byref(mode), C.byref(launches))
         names = ("knn", "symmetrize", "init", "optimize", "total", "knn_tiles", "knn_rerank",
                  "sgd_kernel")
         return {"stage_ms": {n: ms[i] for i, n in enumerate(names)}, "n_edges": e.value,
-                "Q": q.value, "chains": c.value}
+                "Q": q.value, "chains": c.value, "sgd_mode": mode.value, "sgd_launches": launches.value}
 
     def __del__(self):
         # at interpreter shutdown the module globals may already be gone: nothing left to free then

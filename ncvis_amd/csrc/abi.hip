@@ -24,7 +24,8 @@ struct ncvis_handle {
     std::vector<uint64_t> n_noise;
     int dist = DIST_L2;
     int precision = PREC_FP16;
-    int atomic_updates = -1;  // -1: atomics unless a single chain
+    int atomic_updates = -1;  // -1: atomics unless a single chain (per-XCD copies by size), 0 plain
+                              // stores, 1 device atomics, 2 per-XCD copies of Y
     int64_t chains_override = 0;
     int exact_threads = 0;  // 1: exactly n_threads chains (chain t = the reference's thread t)
     int use_graph = 1;      // fit_transform_many: capture the batch in a hipGraph and replay it
@@ -37,6 +38,8 @@ struct ncvis_handle {
     int64_t n_edges = 0;
     float final_Q = 0.f;
     int64_t chains_used = 0;
+    int sgd_mode = 0;
+    int sgd_launches = 0;
 };
 
 static thread_local std::string g_last_error;
@@ -89,7 +92,9 @@ static ChainRule rule_of(const ncvis_handle *h) {
         r.fixed = h->chains_override;
     else if (h->exact_threads && h->n_threads > 0)
         r.fixed = (int64_t)h->n_threads;
-    r.atomic_opt = h->atomic_updates;
+    // atomic_updates 2 asks for the per-XCD copies (a many-chain mode: no lost update across XCDs)
+    r.atomic_opt = h->atomic_updates == 2 ? 1 : h->atomic_updates;
+    r.replicas = h->atomic_updates == 2 ? 1 : (h->atomic_updates < 0 ? tuning().sgd_replicas : 0);
     return r;
 }
 
@@ -120,7 +125,7 @@ struct FitEvents {
     hipEvent_t stage[5];  // start, after knn, after symmetrize, after init, end
     KnnEvents knn;
     std::vector<hipEvent_t> epochs;
-    FitEvents(int n_epochs) : epochs(2 * (size_t)std::max(n_epochs, 0), nullptr) {
+    FitEvents(int n_launches) : epochs(2 * (size_t)std::max(n_launches, 0), nullptr) {
         for (auto &e : stage) e = nullptr;
         knn = KnnEvents{nullptr, nullptr, nullptr};
         for (auto &e : stage) NCVIS_HIP_CHECK(hipEventCreate(&e));
@@ -256,8 +261,12 @@ struct FitBatch {
                      o_dists = take(sizeof(float) * gnk), o_cdist = take(sizeof(float) * 2 * gnk),
                      o_rpg = take(grows ? sizeof(int64_t) * (grows + 1) : 0),
                      o_colg = take(grows ? sizeof(int32_t) * 2 * gnk : 0);
+        // per-XCD copies of Y for the epochs that may run on them (k_seg_dims decides per data set)
+        const bool reps = d >= 1 && d <= 4 && rule.replicas != 0 && rule.atomic_opt != 0;
+        tot.replicas = reps;
+        tot.sub_rows = true;
         struct Off {
-            size_t row_ptr, col, y2, part, stats, chain_row, rng, qpart, qdev;
+            size_t row_ptr, col, y2, part, stats, chain_row, rng, qpart, qdev, yrep, sub_row;
         };
         std::vector<Off> off(B);
         for (int p = 0; p < B; ++p) {
@@ -281,6 +290,8 @@ struct FitBatch {
             off[p].rng = take(sizeof(U128) * v.T_cap);
             off[p].qpart = take(sizeof(float) * v.T_cap);
             off[p].qdev = take(sizeof(float) * (1 + (size_t)std::max(h->n_epochs, 0)));
+            off[p].yrep = take(reps ? sizeof(float) * kXcds * N * d : 0);
+            off[p].sub_row = take(sizeof(int64_t) * v.T_cap);
         }
         const size_t o_errs = take(sizeof(int) * (B + groups.size())), o_segs = take(sizeof(FitSeg) * B);
         arena.alloc(o);
@@ -306,6 +317,8 @@ struct FitBatch {
             v.rng = (U128 *)(w + off[p].rng);
             v.qpart = (float *)(w + off[p].qpart);
             v.qdev = (float *)(w + off[p].qdev);
+            v.Yrep = reps ? (float *)(w + off[p].yrep) : nullptr;
+            v.sub_row = (int64_t *)(w + off[p].sub_row);
         }
     }
 
@@ -410,6 +423,8 @@ struct FitStats {
     int64_t n_edges = 0;
     float Q = 0.f;
     int64_t chains = 0;
+    int mode = 0;
+    int launches = 0;  // epoch kernel launches (epochs x sub-epochs)
 };
 
 // One whole fit (ncvis.cpp:354-459) on device buffers, queued on `st`, timed stage by stage.
@@ -427,6 +442,10 @@ static void optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const 
     for (int i = 0; i < n_epochs; ++i)
         if ((uint64_t)n_noise[i] >= 0xFFFFFFFFull) throw std::invalid_argument("[ncvis_amd] n_noise too large");
     if (N < 2) throw std::invalid_argument("[ncvis_amd] optimize: N must be >= 2");
+    if (atomic_updates < -1 || atomic_updates > 2) throw std::invalid_argument("[ncvis_amd] atomic_updates: -1, 0, 1 or 2");
+    if (atomic_updates == 2 && ws)
+        throw std::invalid_argument("[ncvis_amd] atomic_updates 2 (per-XCD copies of Y) allocates; not available here");
+    if (atomic_updates == 2 && d > 4) throw std::invalid_argument("[ncvis_amd] atomic_updates 2 needs d <= 4");
     check_graph(d_row_ptr, (int64_t)N, d_col, n_edges, nullptr, 0, st, d_err);
     SgdParams p;
     p.a = a;
@@ -443,10 +462,10 @@ static void optimize_device(size_t N, size_t d, const int64_t *d_row_ptr, const 
 
 // ---- scratch of ncvis_optimize_device_noalloc ---------------------------------------------
 // One file-scope device array holds the epoch loop's scratch for callers that cannot allocate:
-// pcg64 states, chain start rows and Q copies for up to kNoAllocMaxChains chains, the per-epoch
-// Q buffer for up to kNoAllocMaxEpochs epochs, the graph-check flag and the data set's
-// descriptor (3.75 MiB per device,
-// held until the process ends). Every call on a device shares that device's copy.
+// pcg64 states, chain start rows, Q copies and sub-epoch rows for up to kNoAllocMaxChains chains,
+// the per-epoch Q buffer for up to kNoAllocMaxEpochs epochs, the graph-check flag and the data
+// set's descriptor (4.75 MiB per device, held until the process ends). Every call on a device
+// shares that device's copy.
 constexpr int64_t kNoAllocMaxChains = 1 << 17;
 constexpr int kNoAllocMaxEpochs = 1 << 16;
 constexpr size_t kAlign = 256;
@@ -457,7 +476,8 @@ constexpr size_t kOffQpart = align_up(kOffChainRow + sizeof(int64_t) * kNoAllocM
 constexpr size_t kOffQdev = align_up(kOffQpart + sizeof(float) * kNoAllocMaxChains);
 constexpr size_t kOffErr = align_up(kOffQdev + sizeof(float) * (1 + (size_t)kNoAllocMaxEpochs));
 constexpr size_t kOffSeg = align_up(kOffErr + sizeof(int));
-constexpr size_t kScratchBytes = align_up(kOffSeg + sizeof(FitSeg));
+constexpr size_t kOffSubRow = align_up(kOffSeg + sizeof(FitSeg));
+constexpr size_t kScratchBytes = align_up(kOffSubRow + sizeof(int64_t) * kNoAllocMaxChains);
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -505,7 +525,8 @@ static void fit_device(ncvis_handle *h, const float *dX, size_t N, size_t D, flo
     } else {
         fb.plan(h, js, g);
     }
-    FitEvents ev(h->n_epochs);
+    const int launches = h->n_epochs * sgd_sub_epochs(fb.tot);
+    FitEvents ev(launches);
     fb.upload(st);
     fb.enqueue(h, st, &ev);
     fb.check_errors(st, false);  // synchronises st
@@ -522,10 +543,12 @@ static void fit_device(ncvis_handle *h, const float *dX, size_t N, size_t D, flo
     NCVIS_HIP_CHECK(hipEventElapsedTime(&t2, ev.knn.tiles_end, ev.knn.rerank_end));
     out.ms[5] = t1;
     out.ms[6] = t2;
-    out.ms[7] = epoch_time_ms(ev.epochs.data(), h->n_epochs);
+    out.ms[7] = epoch_time_ms(ev.epochs.data(), launches);
+    out.launches = launches;
     out.n_edges = v.E;
     out.Q = Q;
     out.chains = v.T;
+    out.mode = v.mode;
 }
 
 extern "C" {
@@ -591,6 +614,7 @@ int ncvis_set_option(ncvis_handle *h, const char *key, double value) {
             if (v < PREC_FP32 || v > PREC_FP16) throw std::invalid_argument("[ncvis_amd] knn_precision must be 0, 1 or 2");
             h->precision = v;
         } else if (k == "atomic_updates") {
+            if (value < -1 || value > 2) throw std::invalid_argument("[ncvis_amd] atomic_updates: -1, 0, 1 or 2");
             h->atomic_updates = (int)value;
         } else if (k == "chains") {
             if (value < 0) throw std::invalid_argument("[ncvis_amd] chains must be >= 0");
@@ -618,6 +642,14 @@ int ncvis_tune(const char *key, double value) {
             t.sgd_plain_far = value != 0;
         } else if (k == "sgd_fast_math") {
             t.sgd_fast_math = value != 0;
+        } else if (k == "sgd_replicas") {
+            if (value != 0 && value != 1) throw std::invalid_argument("[ncvis_amd] sgd_replicas: 0 or 1");
+            t.sgd_replicas = (int)value;
+        } else if (k == "sgd_edge_flush") {
+            t.sgd_edge_flush = value != 0;
+        } else if (k == "sgd_sync") {
+            if (value < 1 || value > 64) throw std::invalid_argument("[ncvis_amd] sgd_sync: 1..64 sub-epochs");
+            t.sgd_sync = (int)value;
         } else if (k == "edges_per_chain") {
             if (value < 1) throw std::invalid_argument("[ncvis_amd] edges_per_chain must be >= 1");
             t.edges_per_chain = (int64_t)value;
@@ -656,6 +688,8 @@ int ncvis_fit_transform_device(ncvis_handle *h, const float *dX, size_t N, size_
         h->n_edges = fs.n_edges;
         h->final_Q = fs.Q;
         h->chains_used = fs.chains;
+        h->sgd_mode = fs.mode;
+        h->sgd_launches = fs.launches;
     });
 }
 
@@ -727,6 +761,13 @@ int ncvis_get_stats(const ncvis_handle *h, double *stage_ms8, int64_t *n_edges, 
     return NCVIS_OK;
 }
 
+int ncvis_get_sgd_mode(const ncvis_handle *h, int *mode, int *launches) {
+    if (!h) return NCVIS_EINVAL;
+    if (mode) *mode = h->sgd_mode;
+    if (launches) *launches = h->sgd_launches;
+    return NCVIS_OK;
+}
+
 int ncvis_knn_device(const float *dX, size_t N, size_t D, size_t k, int dist, int precision, int32_t *d_inds,
                      float *d_dists, void *stream) {
     return guarded(g_last_error, [&] {
@@ -787,7 +828,7 @@ int ncvis_optimize_device_noalloc(size_t N, size_t d, const int64_t *d_row_ptr, 
                                         " epochs exceed the scratch limit of " + std::to_string(kNoAllocMaxEpochs));
         unsigned char *base = noalloc_scratch((hipStream_t)stream);
         SgdWorkspace ws{(int64_t *)(base + kOffChainRow), (U128 *)(base + kOffRng), (float *)(base + kOffQpart),
-                        (float *)(base + kOffQdev), (FitSeg *)(base + kOffSeg)};
+                        (float *)(base + kOffQdev), (FitSeg *)(base + kOffSeg), (int64_t *)(base + kOffSubRow)};
         optimize_device(N, d, d_row_ptr, d_col, n_edges, dY, Q, n_epochs, n_noise, a, b, alpha, alpha_Q, seed, T,
                         atomic_updates, q_trace, (hipStream_t)stream, &ws, (int *)(base + kOffErr));
     });

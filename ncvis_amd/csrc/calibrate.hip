@@ -1,13 +1,15 @@
-// Calibration of the random read-modify-write rate that bounds the many-chain SGD epoch.
+// Calibration of the random read-modify-write rates that bound the many-chain SGD epochs.
 //
-// Every SGD sample reads one random 8-byte row of Y (d = 2) and updates it with one device-scope
-// 64-bit compare-and-swap (the far endpoint; the source row stays in registers). The roofline of
-// that kernel is therefore the rate at which the GPU completes random load + CAS pairs on a table
-// of Y's size, not HBM bandwidth: a CAS on a line shared by the XCDs' separate L2 caches is
-// performed at the memory side. k_rmw_probe issues exactly that pattern (threads x iters random
-// rows, pcg64-drawn, the same launch shape as the epoch kernel) and ncvis_calibrate_rmw reports
-// rows per second for load-only and for load + CAS, so the measured epoch rate can be stated as a
-// fraction of the pattern's own ceiling.
+// Every SGD sample reads one random 8-byte row of Y (d = 2) and updates it (the far endpoint; the
+// source row stays in registers). With device atomics (mode 2) the update is a device-scope 64-bit
+// compare-and-swap, performed at the memory side because the XCDs' L2 caches are not coherent with
+// each other; on per-XCD copies of Y (mode 3) it is a second read and a plain store of the XCD's
+// own copy, completed in that XCD's L2. The roofline of either kernel is the rate at which the GPU
+// completes that pattern on a table of Y's size, not HBM bandwidth. k_rmw_probe issues exactly
+// those patterns (threads x iters random rows, pcg64-drawn, the same launch shape as the epoch
+// kernel); ncvis_calibrate_rmw reports rows per second for load-only and for load + CAS,
+// ncvis_calibrate_rmw_copies for the per-XCD-copy update, so the measured epoch rate can be
+// stated as a fraction of its pattern's own ceiling.
 #include <algorithm>
 #include <cstdint>
 #include <stdexcept>
@@ -19,17 +21,36 @@
 
 namespace ncvis_hip {
 
-template <int CAS>
+// MODE 0: load only; 1: load + device-scope 64-bit CAS on one table (the device-atomic epoch);
+// 2: the per-XCD-copy epoch's update: the table holds kXcds copies, a wave works on its XCD's
+// copy (hardware XCC id), reads the row, reads it again and stores the update, all at workgroup
+// scope (served by the XCD's L2).
+template <int MODE>
 __global__ __launch_bounds__(256) void k_rmw_probe(unsigned long long *table, uint64_t rows, int iters,
                                                    float *sink) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     U128 g = pcg_seed(777 + (uint64_t)t);
     float acc = 0.f;
+    unsigned long long *tab = table;
+    if (MODE == 2) {
+        const int x = __builtin_amdgcn_s_getreg((20 /* HW_REG_XCC_ID */) | (0 << 6) | (15 << 11)) & (kXcds - 1);
+        tab = table + (uint64_t)x * rows;
+    }
     for (int it = 0; it < iters; ++it) {
         const uint64_t r = uniform_index(g, rows);
-        unsigned long long *addr = table + r;
+        unsigned long long *addr = tab + r;
+        if (MODE == 2) {
+            const unsigned long long first = __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            acc += __builtin_bit_cast(float2, first).x;
+            const unsigned long long cur = __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            float2 v = __builtin_bit_cast(float2, cur);
+            float2 nv = make_float2(__fsub_rn(v.x, 1e-7f), __fadd_rn(v.y, 1e-7f));
+            __hip_atomic_store(addr, __builtin_bit_cast(unsigned long long, nv), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+            continue;
+        }
         unsigned long long expect = *addr;
-        if (CAS) {
+        if (MODE == 1) {
             while (true) {
                 float2 v = __builtin_bit_cast(float2, expect);
                 float2 nv = make_float2(__fsub_rn(v.x, 1e-7f), __fadd_rn(v.y, 1e-7f));
@@ -55,52 +76,54 @@ using namespace ncvis_hip;
 
 static thread_local std::string g_cal_error;
 
-extern "C" {
-
-const char *ncvis_calibrate_last_error(void) { return g_cal_error.c_str(); }
-
-int ncvis_calibrate_rmw(size_t rows, int64_t threads, int iters, double *load_rows_per_s, double *cas_rows_per_s) {
-    try {
-        if (rows < 1 || threads < 256 || iters < 1) throw std::invalid_argument("[ncvis_amd] calibrate: bad sizes");
-        hipStream_t st = nullptr;
-        NCVIS_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        struct StreamGuard {
-            hipStream_t s;
-            ~StreamGuard() { (void)hipStreamDestroy(s); }
-        } sg{st};
-        DevBuf<unsigned long long> table(rows);
-        DevBuf<float> sink(1);
-        k_rmw_fill<<<cdiv((int64_t)rows, 256), 256, 0, st>>>(table.p, rows);
-        NCVIS_LAUNCH_CHECK();
-        hipEvent_t e0, e1;
-        NCVIS_HIP_CHECK(hipEventCreate(&e0));
-        NCVIS_HIP_CHECK(hipEventCreate(&e1));
-        struct EvGuard {
-            hipEvent_t a, b;
-            ~EvGuard() {
-                (void)hipEventDestroy(a);
-                (void)hipEventDestroy(b);
-            }
-        } eg{e0, e1};
-        const dim3 grid((unsigned)(threads / 256));
-        double rate[2] = {0.0, 0.0};
-        for (int cas = 0; cas < 2; ++cas) {
-            for (int rep = 0; rep < 3; ++rep) {  // the first run warms up; the best of two is kept
-                NCVIS_HIP_CHECK(hipEventRecord(e0, st));
-                if (cas)
-                    k_rmw_probe<1><<<grid, 256, 0, st>>>(table.p, rows, iters, sink.p);
-                else
-                    k_rmw_probe<0><<<grid, 256, 0, st>>>(table.p, rows, iters, sink.p);
-                NCVIS_LAUNCH_CHECK();
-                NCVIS_HIP_CHECK(hipEventRecord(e1, st));
-                NCVIS_HIP_CHECK(hipEventSynchronize(e1));
-                float ms = 0.f;
-                NCVIS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                if (rep > 0) rate[cas] = std::max(rate[cas], (double)(threads / 256 * 256) * iters / (ms * 1e-3));
-            }
+// Best of two timed runs (after one warm-up) of k_rmw_probe<MODE>: rows per second.
+static double probe_rate(int mode, size_t rows, int64_t threads, int iters) {
+    if (rows < 1 || threads < 256 || iters < 1) throw std::invalid_argument("[ncvis_amd] calibrate: bad sizes");
+    hipStream_t st = nullptr;
+    NCVIS_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{st};
+    const size_t entries = rows * (mode == 2 ? kXcds : 1);
+    DevBuf<unsigned long long> table(entries);
+    DevBuf<float> sink(1);
+    k_rmw_fill<<<cdiv((int64_t)entries, 256), 256, 0, st>>>(table.p, entries);
+    NCVIS_LAUNCH_CHECK();
+    hipEvent_t e0, e1;
+    NCVIS_HIP_CHECK(hipEventCreate(&e0));
+    NCVIS_HIP_CHECK(hipEventCreate(&e1));
+    struct EvGuard {
+        hipEvent_t a, b;
+        ~EvGuard() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
         }
-        if (load_rows_per_s) *load_rows_per_s = rate[0];
-        if (cas_rows_per_s) *cas_rows_per_s = rate[1];
+    } eg{e0, e1};
+    const dim3 grid((unsigned)(threads / 256));
+    double rate = 0.0;
+    for (int rep = 0; rep < 3; ++rep) {  // the first run warms up; the best of two is kept
+        NCVIS_HIP_CHECK(hipEventRecord(e0, st));
+        if (mode == 0)
+            k_rmw_probe<0><<<grid, 256, 0, st>>>(table.p, rows, iters, sink.p);
+        else if (mode == 1)
+            k_rmw_probe<1><<<grid, 256, 0, st>>>(table.p, rows, iters, sink.p);
+        else
+            k_rmw_probe<2><<<grid, 256, 0, st>>>(table.p, rows, iters, sink.p);
+        NCVIS_LAUNCH_CHECK();
+        NCVIS_HIP_CHECK(hipEventRecord(e1, st));
+        NCVIS_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        NCVIS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (rep > 0) rate = std::max(rate, (double)(threads / 256 * 256) * iters / (ms * 1e-3));
+    }
+    return rate;
+}
+
+template <typename F>
+static int cal_guarded(F &&f) {
+    try {
+        f();
         g_cal_error.clear();
         return NCVIS_OK;
     } catch (const HipError &e) {
@@ -110,6 +133,26 @@ int ncvis_calibrate_rmw(size_t rows, int64_t threads, int iters, double *load_ro
         g_cal_error = e.what();
         return NCVIS_EINVAL;
     }
+}
+
+extern "C" {
+
+const char *ncvis_calibrate_last_error(void) { return g_cal_error.c_str(); }
+
+int ncvis_calibrate_rmw(size_t rows, int64_t threads, int iters, double *load_rows_per_s, double *cas_rows_per_s) {
+    return cal_guarded([&] {
+        const double l = probe_rate(0, rows, threads, iters);
+        const double c = probe_rate(1, rows, threads, iters);
+        if (load_rows_per_s) *load_rows_per_s = l;
+        if (cas_rows_per_s) *cas_rows_per_s = c;
+    });
+}
+
+int ncvis_calibrate_rmw_copies(size_t rows, int64_t threads, int iters, double *rows_per_s) {
+    return cal_guarded([&] {
+        const double r = probe_rate(2, rows, threads, iters);
+        if (rows_per_s) *rows_per_s = r;
+    });
 }
 
 }  // extern "C"

@@ -70,7 +70,9 @@ __global__ void k_seg_dims(FitSeg *__restrict__ s, int B, ChainRule rule, int d)
     const int atomic = rule.atomic_opt >= 0 ? rule.atomic_opt : (T > 1 ? 1 : 0);
     g.E = E;
     g.T = T;
-    g.mode = atomic ? ((d >= 1 && d <= 4 && (d % 2 != 0 || g.pairs_ok)) ? 2 : 1) : 0;
+    int mode = atomic ? ((d >= 1 && d <= 4 && (d % 2 != 0 || g.pairs_ok)) ? 2 : 1) : 0;
+    if (atomic && rule.replicas && g.Yrep && d >= 1 && d <= 4) mode = 3;
+    g.mode = mode;
 }
 
 void seg_dims(FitSeg *d_segs, int B, const ChainRule &rule, int d, hipStream_t st) {
@@ -304,6 +306,7 @@ struct SampleConst {
     float noise_f;     // (float)cur_noise
     uint32_t noise;    // cur_noise
     int fast_math;     // 1: single-precision powf / expf
+    int edge_flush;    // mode 3: write the source row's accumulated update after every edge
 };
 
 // The sample body of ncvis.cpp:304-339 on a register copy of the two rows. Returns the weight
@@ -430,7 +433,7 @@ __global__ void k_chain_rows_seg(const FitSeg *__restrict__ s, int B, int64_t sl
 // (ncvis.cpp:290-340). DIM 0: d only known at run time.
 template <int DIM>
 __device__ __forceinline__ void chain_exact(float *__restrict__ Y, int d_rt, const int64_t *__restrict__ row_ptr,
-                                            const int32_t *__restrict__ col, int64_t s0, int64_t e0, int64_t r,
+                                            const int32_t *__restrict__ col, int64_t s0, int64_t e0, int64_t &r,
                                             U128 &g, float &Qc, const SampleConst &c, uint64_t N, int atomic_mode) {
     int64_t rend = row_ptr[r + 1];
     for (int64_t e = s0; e < e0; ++e) {
@@ -456,36 +459,99 @@ __device__ __forceinline__ void chain_exact(float *__restrict__ Y, int d_rt, con
     }
 }
 
-// One chain of the many-chain (Hogwild) mode. The chain keeps the row of its current source
+// Row access of the per-XCD copies (mode 3): relaxed atomic loads and stores at workgroup scope
+// (plain loads/stores that skip the CU's L1, so a row is read from the XCD's L2, where every
+// update of that copy lands), coordinate pairs as one 8-byte access.
+template <int DIM>
+__device__ __forceinline__ void rep_load(const float *p, float (&v)[DIM]) {
+    if constexpr (DIM % 2 == 0) {
+#pragma unroll
+        for (int k = 0; k < DIM; k += 2) {
+            const unsigned long long u = __hip_atomic_load((const unsigned long long *)(p + k), __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+            const float2 f = __builtin_bit_cast(float2, u);
+            v[k] = f.x;
+            v[k + 1] = f.y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) v[k] = __hip_atomic_load(p + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void rep_store(float *p, const float (&v)[DIM]) {
+    if constexpr (DIM % 2 == 0) {
+#pragma unroll
+        for (int k = 0; k < DIM; k += 2)
+            __hip_atomic_store((unsigned long long *)(p + k),
+                               __builtin_bit_cast(unsigned long long, make_float2(v[k], v[k + 1])), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) __hip_atomic_store(p + k, v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+// One chain of the many-chain (Hogwild) modes. The chain keeps the row of its current source
 // point in registers and adds its own updates of that row there (it sees its own writes, as a
-// thread of the reference does), flushing them to Y with atomics when it moves to the next row;
-// updates of the other endpoint go to Y at once with atomics, so concurrent chains lose no
-// update. The negatives of an edge are drawn before their rows are read (the draw sequence does
-// not depend on Y), so PF rows are fetched together instead of one dependent load per sample.
-template <int DIM, int PF>
+// thread of the reference does), flushing them to Y when it moves to the next row; updates of the
+// other endpoint go to Y at once. The negatives of an edge are drawn before their rows are read
+// (the draw sequence does not depend on Y), so PF rows are fetched together instead of one
+// dependent load per sample.
+//   REP = false (mode 2): Y is shared by all XCDs; updates are device-scope atomics (a 64-bit
+//     compare-and-swap per coordinate pair, float atomics for odd d), performed at the memory
+//     side, so concurrent chains lose no update.
+//   REP = true (mode 3): Y is this XCD's own copy, touched by no other XCD, so plain
+//     read-modify-writes complete in the XCD's L2; the far row is read again right before it is
+//     written, so only a chain of the same XCD updating that row in between loses its update
+//     (the reference's Hogwild race, narrowed to one L2 round trip). The copies are merged after
+//     every sub-epoch (k_rep_sync).
+template <int DIM, int PF, bool REP>
 __device__ __forceinline__ void chain_hw(float *__restrict__ Y, const int64_t *__restrict__ row_ptr,
-                                         const int32_t *__restrict__ col, int64_t s0, int64_t e0, int64_t r,
+                                         const int32_t *__restrict__ col, int64_t s0, int64_t e0, int64_t &r,
                                          U128 &g, float &Qc, const SampleConst &c, uint64_t N, int plain_far) {
     int64_t rend = row_ptr[r + 1];
     float base[DIM], acc[DIM];
+    auto flush = [&](int64_t row) {
+        if constexpr (REP) {
+            bool any = false;
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) {
-        base[k] = Y[r * DIM + k];
-        acc[k] = 0.f;
-    }
+            for (int k = 0; k < DIM; ++k) any |= acc[k] != 0.f;
+            if (any) {
+                float cur[DIM];
+                rep_load<DIM>(Y + row * DIM, cur);
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) cur[k] = __fadd_rn(cur[k], acc[k]);
+                rep_store<DIM>(Y + row * DIM, cur);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < DIM; ++k)
+                if (acc[k] != 0.f)
+                    __hip_atomic_fetch_add(&Y[row * DIM + k], acc[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    };
+    auto fetch = [&](int64_t row, float (&v)[DIM]) {
+        if constexpr (REP) {
+            rep_load<DIM>(Y + row * DIM, v);
+        } else {
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) v[k] = Y[row * DIM + k];
+        }
+    };
+    fetch(r, base);
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) acc[k] = 0.f;
     const uint32_t total = c.noise + 1;
     for (int64_t e = s0; e < e0; ++e) {
         while (e >= rend) {
-#pragma unroll
-            for (int k = 0; k < DIM; ++k)
-                if (acc[k] != 0.f) __hip_atomic_fetch_add(&Y[r * DIM + k], acc[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            flush(r);
             ++r;
             rend = row_ptr[r + 1];
+            fetch(r, base);
 #pragma unroll
-            for (int k = 0; k < DIM; ++k) {
-                base[k] = Y[r * DIM + k];
-                acc[k] = 0.f;
-            }
+            for (int k = 0; k < DIM; ++k) acc[k] = 0.f;
         }
         const int64_t id = r;
         for (uint32_t j0 = 0; j0 < total; j0 += PF) {
@@ -508,10 +574,7 @@ __device__ __forceinline__ void chain_hw(float *__restrict__ Y, const int64_t *_
                 }
             }
 #pragma unroll
-            for (int p = 0; p < PF; ++p) {
-#pragma unroll
-                for (int k = 0; k < DIM; ++k) yo[p][k] = Y[oth[p] * DIM + k];
-            }
+            for (int p = 0; p < PF; ++p) fetch(oth[p], yo[p]);
 #pragma unroll
             for (int p = 0; p < PF; ++p) {
                 if (j0 + p < total) {
@@ -521,7 +584,13 @@ __device__ __forceinline__ void chain_hw(float *__restrict__ Y, const int64_t *_
                     sgd_math<DIM>(yi, yo[p], (j0 + p) == 0, c, Qc, dx);
 #pragma unroll
                     for (int k = 0; k < DIM; ++k) acc[k] = __fadd_rn(acc[k], dx[k]);
-                    if (plain_far) {
+                    if constexpr (REP) {
+                        float cur[DIM];
+                        rep_load<DIM>(Y + oth[p] * DIM, cur);
+#pragma unroll
+                        for (int k = 0; k < DIM; ++k) cur[k] = __fsub_rn(cur[k], dx[k]);
+                        rep_store<DIM>(Y + oth[p] * DIM, cur);
+                    } else if (plain_far) {
 #pragma unroll
                         for (int k = 0; k < DIM; ++k) Y[oth[p] * DIM + k] = __fsub_rn(yo[p][k], dx[k]);
                     } else if constexpr (DIM % 2 == 0) {
@@ -551,35 +620,65 @@ __device__ __forceinline__ void chain_hw(float *__restrict__ Y, const int64_t *_
                 }
             }
         }
-    }
+        if constexpr (REP) {
+            if (c.edge_flush) {
+                // the source row's update goes out with every edge (a lost write loses one edge's
+                // samples, not the row's whole run) and the chain picks up the others' updates of it
+                bool any = false;
 #pragma unroll
-    for (int k = 0; k < DIM; ++k)
-        if (acc[k] != 0.f) __hip_atomic_fetch_add(&Y[r * DIM + k], acc[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int k = 0; k < DIM; ++k) any |= acc[k] != 0.f;
+                if (any) {
+                    float cur[DIM];
+                    rep_load<DIM>(Y + r * DIM, cur);
+#pragma unroll
+                    for (int k = 0; k < DIM; ++k) {
+                        cur[k] = __fadd_rn(cur[k], acc[k]);
+                        base[k] = cur[k];
+                        acc[k] = 0.f;
+                    }
+                    rep_store<DIM>(Y + r * DIM, cur);
+                }
+            }
+        }
+    }
+    flush(r);
 }
 
-// One epoch of every chain of every data set: thread = chain slot (slots past a data set's T
-// exit). DIM 1..4: d at compile time (mode 2 available); DIM 0: any d (modes 0 and 1).
+// Hardware id of the XCD (L2 domain) this wave runs on.
+__device__ __forceinline__ int xcd_id() {
+    return __builtin_amdgcn_s_getreg((20 /* HW_REG_XCC_ID */) | (0 << 6) | (15 << 11)) & (kXcds - 1);
+}
+
+// Sub-epoch `sub` of `S` of every chain of every data set: thread = chain slot (slots past a
+// data set's T exit); chain t takes part sub of its static edge chunk, so S consecutive launches
+// process the chunk in the reference's order (pcg64 state, Q copy and current row carried over).
+// DIM 1..4: d at compile time (modes 2 and 3 available); DIM 0: any d (modes 0 and 1).
 template <int DIM, int PF>
 __global__ __launch_bounds__(256) void k_sgd_epoch_seg(const FitSeg *__restrict__ s, int B, int64_t slots,
-                                                       SampleConst c, int d_rt, int plain_far) {
+                                                       SampleConst c, int d_rt, int plain_far, int sub, int S) {
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gt >= slots) return;
     const FitSeg &g = s[seg_find<1>(s, B, gt)];
     const int64_t t = gt - g.chain_off;
     const int64_t T = g.T;
     if (t >= T) return;
-    int64_t s0, e0;
-    static_chunk(g.E, T, t, s0, e0);
-    float Qc = g.qdev[0];
+    int64_t c0, c1;
+    static_chunk(g.E, T, t, c0, c1);
+    const int64_t len = c1 - c0;
+    const int64_t s0 = c0 + len * sub / S, e0 = c0 + len * (sub + 1) / S;
+    float Qc = sub == 0 ? g.qdev[0] : g.qpart[t];
     U128 rng = g.rng[t];
-    const int64_t r = g.chain_row[t];
+    int64_t r = sub == 0 ? g.chain_row[t] : g.sub_row[t];
     const uint64_t N = (uint64_t)g.N;
     const int mode = g.mode;
     SampleConst ce = c;
     ce.fast_math = 0;  // the exact modes keep the reference's rounding whatever the knob says
     if constexpr (DIM >= 1 && DIM <= 4) {
-        if (mode == 2)
-            chain_hw<DIM, PF>(g.Y, g.row_ptr, g.col, s0, e0, r, rng, Qc, c, N, plain_far);
+        if (mode == 3)
+            chain_hw<DIM, PF, true>(g.Yrep + (int64_t)xcd_id() * g.N * DIM, g.row_ptr, g.col, s0, e0, r, rng, Qc, c,
+                                    N, 0);
+        else if (mode == 2)
+            chain_hw<DIM, PF, false>(g.Y, g.row_ptr, g.col, s0, e0, r, rng, Qc, c, N, plain_far);
         else
             chain_exact<DIM>(g.Y, DIM, g.row_ptr, g.col, s0, e0, r, rng, Qc, ce, N, mode);
     } else {
@@ -587,6 +686,43 @@ __global__ __launch_bounds__(256) void k_sgd_epoch_seg(const FitSeg *__restrict_
     }
     g.rng[t] = rng;
     g.qpart[t] = Qc;
+    if (S > 1) g.sub_row[t] = r;  // S > 1 only with sub_row (seg_optimize)
+}
+
+// Per-XCD copies of Y (mode 3 data sets): init = 1 copies Y into every copy; otherwise the copies
+// are merged, Y + sum over XCDs of (copy - Y) in a fixed order (every update of every copy kept
+// once), and the result is written to Y and to every copy. One thread per row.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_rep_sync(const FitSeg *__restrict__ s, int B, int64_t rows, int init) {
+    const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= rows) return;
+    const FitSeg &g = s[seg_find<0>(s, B, gi)];
+    if (g.mode != 3) return;
+    const int64_t i = gi - g.row_off;
+    const int64_t stride = g.N * DIM;
+    float y[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) y[k] = g.Y[i * DIM + k];
+    if (!init) {
+        float v[DIM];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) v[k] = y[k];
+#pragma unroll
+        for (int x = 0; x < kXcds; ++x) {
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) v[k] = __fadd_rn(v[k], __fsub_rn(g.Yrep[x * stride + i * DIM + k], y[k]));
+        }
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            y[k] = v[k];
+            g.Y[i * DIM + k] = v[k];
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < kXcds; ++x) {
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) g.Yrep[x * stride + i * DIM + k] = y[k];
+    }
 }
 
 // One 1024-thread block per data set; each thread keeps four loads in flight so 2^17 chain
@@ -625,22 +761,39 @@ __global__ void k_seg_q0(const FitSeg *__restrict__ s, int B, float Q0) {
 
 template <int DIM>
 static void launch_epoch_seg(int PF, dim3 grid, const FitSeg *segs, int B, int64_t slots, const SampleConst &c,
-                             int d, int plain_far, hipStream_t st) {
+                             int d, int plain_far, int sub, int S, hipStream_t st) {
     switch (PF) {
-        case 2: k_sgd_epoch_seg<DIM, 2><<<grid, 256, 0, st>>>(segs, B, slots, c, d, plain_far); break;
-        case 4: k_sgd_epoch_seg<DIM, 4><<<grid, 256, 0, st>>>(segs, B, slots, c, d, plain_far); break;
-        case 8: k_sgd_epoch_seg<DIM, 8><<<grid, 256, 0, st>>>(segs, B, slots, c, d, plain_far); break;
-        default: k_sgd_epoch_seg<DIM, 1><<<grid, 256, 0, st>>>(segs, B, slots, c, d, plain_far); break;
+        case 2: k_sgd_epoch_seg<DIM, 2><<<grid, 256, 0, st>>>(segs, B, slots, c, d, plain_far, sub, S); break;
+        case 4: k_sgd_epoch_seg<DIM, 4><<<grid, 256, 0, st>>>(segs, B, slots, c, d, plain_far, sub, S); break;
+        case 8: k_sgd_epoch_seg<DIM, 8><<<grid, 256, 0, st>>>(segs, B, slots, c, d, plain_far, sub, S); break;
+        default: k_sgd_epoch_seg<DIM, 1><<<grid, 256, 0, st>>>(segs, B, slots, c, d, plain_far, sub, S); break;
     }
 }
+
+static void rep_sync(const FitSeg *segs, int B, int64_t rows, int d, int init, hipStream_t st) {
+    const int nb = (int)cdiv(rows, 256);
+    switch (d) {
+        case 1: k_rep_sync<1><<<nb, 256, 0, st>>>(segs, B, rows, init); break;
+        case 2: k_rep_sync<2><<<nb, 256, 0, st>>>(segs, B, rows, init); break;
+        case 3: k_rep_sync<3><<<nb, 256, 0, st>>>(segs, B, rows, init); break;
+        case 4: k_rep_sync<4><<<nb, 256, 0, st>>>(segs, B, rows, init); break;
+        default: return;  // mode 3 needs d <= 4
+    }
+    NCVIS_LAUNCH_CHECK();
+}
+
+int sgd_sub_epochs(const SegTotals &tot) { return tot.sub_rows ? std::max(1, tuning().sgd_sync) : 1; }
 
 void seg_optimize(const FitSeg *d_segs, int B, const SegTotals &tot, int d, float Q0, const SgdParams &p,
                   hipStream_t st, hipEvent_t *epoch_events) {
     if (tot.slots <= 0) return;
     const Tuning &tu = tuning();
+    const bool rep = tot.replicas && d >= 1 && d <= 4;
+    const int S = sgd_sub_epochs(tot);
     k_seg_q0<<<cdiv(B, 64), 64, 0, st>>>(d_segs, B, Q0);
     k_chain_rows_seg<<<cdiv(tot.slots, 256), 256, 0, st>>>(d_segs, B, tot.slots, p.seed);
     NCVIS_LAUNCH_CHECK();
+    if (rep) rep_sync(d_segs, B, tot.rows, d, 1, st);
     const dim3 grid((unsigned)cdiv(tot.slots, 256));
     for (int epoch = 0; epoch < p.n_epochs; ++epoch) {
         SampleConst c;
@@ -652,18 +805,23 @@ void seg_optimize(const FitSeg *d_segs, int B, const SegTotals &tot, int d, floa
         c.noise = (uint32_t)p.n_noise[epoch];
         c.noise_f = (float)p.n_noise[epoch];
         c.fast_math = tu.sgd_fast_math;
-        if (epoch_events) NCVIS_HIP_CHECK(hipEventRecord(epoch_events[2 * epoch], st));
-        switch (d) {
-            case 1: launch_epoch_seg<1>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, st); break;
-            case 2: launch_epoch_seg<2>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, st); break;
-            case 3: launch_epoch_seg<3>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, st); break;
-            case 4: launch_epoch_seg<4>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, st); break;
-            default:
-                k_sgd_epoch_seg<0, 1><<<grid, 256, 0, st>>>(d_segs, B, tot.slots, c, d, tu.sgd_plain_far);
-                break;
+        c.edge_flush = tu.sgd_edge_flush;
+        for (int sub = 0; sub < S; ++sub) {
+            hipEvent_t *ev = epoch_events ? epoch_events + 2 * ((size_t)epoch * S + sub) : nullptr;
+            if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev[0], st));
+            switch (d) {
+                case 1: launch_epoch_seg<1>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, sub, S, st); break;
+                case 2: launch_epoch_seg<2>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, sub, S, st); break;
+                case 3: launch_epoch_seg<3>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, sub, S, st); break;
+                case 4: launch_epoch_seg<4>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, sub, S, st); break;
+                default:
+                    k_sgd_epoch_seg<0, 1><<<grid, 256, 0, st>>>(d_segs, B, tot.slots, c, d, tu.sgd_plain_far, sub, S);
+                    break;
+            }
+            NCVIS_LAUNCH_CHECK();
+            if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev[1], st));
+            if (rep) rep_sync(d_segs, B, tot.rows, d, 0, st);
         }
-        NCVIS_LAUNCH_CHECK();
-        if (epoch_events) NCVIS_HIP_CHECK(hipEventRecord(epoch_events[2 * epoch + 1], st));
         k_q_reduce_seg<<<B, 1024, 0, st>>>(d_segs, epoch);
         NCVIS_LAUNCH_CHECK();
     }
@@ -679,6 +837,15 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
     DevBuf<U128> own_rng;
     DevBuf<float> own_qpart, own_qdev;
     DevBuf<FitSeg> own_seg;
+    DevBuf<float> own_rep;
+    DevBuf<int64_t> own_sub;
+    // atomic_updates 2: the many-chain epochs on per-XCD copies of Y (memory of their own)
+    const bool rep = p.atomic_updates == 2;
+    if (rep) {
+        if (d > 4) throw std::invalid_argument("[ncvis_amd] atomic_updates 2 (per-XCD copies of Y) needs d <= 4");
+        if (ws) throw std::invalid_argument("[ncvis_amd] atomic_updates 2 needs device memory for the copies of Y");
+        own_rep.alloc((size_t)kXcds * N * d);
+    }
     SgdWorkspace w;
     if (ws) {
         w = *ws;
@@ -688,7 +855,8 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
         own_qpart.alloc((size_t)T);
         own_qdev.alloc(1 + (size_t)std::max(p.n_epochs, 1));
         own_seg.alloc(1);
-        w = SgdWorkspace{own_chain_row.p, own_rng.p, own_qpart.p, own_qdev.p, own_seg.p};
+        own_sub.alloc((size_t)T);
+        w = SgdWorkspace{own_chain_row.p, own_rng.p, own_qpart.p, own_qdev.p, own_seg.p, own_sub.p};
     }
     FitSeg v{};
     v.Y = dY;
@@ -703,11 +871,15 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
     v.pairs_ok = ((uintptr_t)dY % 8 == 0) ? 1 : 0;
     v.E = E;
     v.T = T;
-    v.mode = p.atomic_updates ? ((d <= 4 && (d % 2 != 0 || v.pairs_ok)) ? 2 : 1) : 0;
+    v.mode = rep ? 3 : (p.atomic_updates ? ((d <= 4 && (d % 2 != 0 || v.pairs_ok)) ? 2 : 1) : 0);
+    v.Yrep = rep ? own_rep.p : nullptr;
+    v.sub_row = w.sub_row;
     seg_set(w.seg, v, st);
     SegTotals tot;
     tot.rows = N;
     tot.slots = T;
+    tot.replicas = rep;
+    tot.sub_rows = w.sub_row != nullptr;
     std::vector<hipEvent_t> ev;
     struct EvGuard {
         std::vector<hipEvent_t> &e;
@@ -715,8 +887,9 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
             for (hipEvent_t x : e) (void)hipEventDestroy(x);
         }
     } evg{ev};
+    const int launches = std::max(p.n_epochs, 0) * sgd_sub_epochs(tot);
     if (epoch_kernel_ms) {
-        ev.resize(2 * (size_t)std::max(p.n_epochs, 0));
+        ev.resize(2 * (size_t)launches);
         for (auto &x : ev) {
             x = nullptr;
             NCVIS_HIP_CHECK(hipEventCreate(&x));
@@ -728,14 +901,14 @@ void optimize_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_t *d_c
         NCVIS_HIP_CHECK(
             hipMemcpyAsync(q_trace, w.qdev + 1, sizeof(float) * p.n_epochs, hipMemcpyDeviceToHost, st));
     NCVIS_HIP_CHECK(hipStreamSynchronize(st));
-    if (epoch_kernel_ms) *epoch_kernel_ms = epoch_time_ms(ev.data(), p.n_epochs);
+    if (epoch_kernel_ms) *epoch_kernel_ms = epoch_time_ms(ev.data(), launches);
 }
 
-double epoch_time_ms(const hipEvent_t *ev, int n_epochs) {
+double epoch_time_ms(const hipEvent_t *ev, int n_launches) {
     double tot = 0.0;
-    for (int epoch = 0; epoch < n_epochs; ++epoch) {
+    for (int l = 0; l < n_launches; ++l) {
         float ms = 0.f;
-        NCVIS_HIP_CHECK(hipEventElapsedTime(&ms, ev[2 * epoch], ev[2 * epoch + 1]));
+        NCVIS_HIP_CHECK(hipEventElapsedTime(&ms, ev[2 * l], ev[2 * l + 1]));
         tot += ms;
     }
     return tot;

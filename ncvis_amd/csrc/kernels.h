@@ -14,6 +14,11 @@ constexpr int kErrSymRange = 2;       // a kNN index outside [0, N)
 struct Tuning {
     int sgd_prefetch = 1;            // rows fetched together per chain in the many-chain epoch (1,2,4,8)
     int sgd_plain_far = 0;           // 1: far-endpoint updates with plain stores instead of atomics
+    int sgd_replicas = 1;            // many-chain epochs on per-XCD copies of Y (mode 3): 1 when the
+                                     // atomics are automatic, 0 never (device atomics, mode 2)
+    int sgd_sync = 16;               // sub-epochs per epoch: every chain writes its source row out and,
+                                     // with copies, the copies are merged after each
+    int sgd_edge_flush = 1;          // per-XCD copies: write the source row's update after every edge
     int sgd_fast_math = 0;           // 1: single-precision powf/expf in the many-chain epoch
     int64_t edges_per_chain = 64;    // auto chain count = E / edges_per_chain,
     int64_t min_chains = 1;          // ... at least min(E / 16, min_chains) (1: off) ...
@@ -138,6 +143,7 @@ struct ChainRule {
     int64_t fixed;
     int64_t edges_per_chain, min_chains, max_chains;
     int atomic_opt;
+    int replicas = 0;  // 1: many-chain epochs on per-XCD copies of Y (when d <= 4), 0: device atomics
 };
 __host__ __device__ inline int64_t chain_rule(int64_t E, const ChainRule &r) {
     if (r.fixed > 0) return r.fixed;
@@ -169,13 +175,22 @@ struct FitSeg {
     int nb;                    // blocks of the init statistics (min(1024, ceil(N / 256)))
     int pairs_ok;              // Y is 8-byte aligned (64-bit compare-and-swap on pairs)
     int64_t E, T;              // edges and chains (k_seg_dims, or the host for stage calls)
-    int mode;                  // 0 exact plain stores, 1 exact with float atomics, 2 many-chain
+    int mode;                  // 0 exact plain stores, 1 exact with float atomics, 2 many-chain with
+                               // device atomics, 3 many-chain on per-XCD copies of Y
+    float *Yrep;               // [kXcds, N, d] per-XCD copies of Y (mode 3), or null
+    int64_t *sub_row;          // [T_cap] row of each chain's last edge between sub-epochs (null: one
+                               // launch per epoch)
 };
 
+// XCDs (L2 domains) of the part; the per-XCD copies of Y are indexed by the hardware XCC id.
+constexpr int kXcds = 8;
+
 struct SegTotals {
-    int64_t rows = 0;   // sum of N
-    int64_t slots = 0;  // sum of T_cap
-    int max_nb = 1;     // largest nb
+    int64_t rows = 0;       // sum of N
+    int64_t slots = 0;      // sum of T_cap
+    int max_nb = 1;         // largest nb
+    bool replicas = false;  // some data set has per-XCD copies of Y (merged after every sub-epoch)
+    bool sub_rows = false;  // every data set has sub_row: epochs may run as several sub-epochs
 };
 
 // Writes one descriptor from the host value (a kernel argument: no copy, capturable).
@@ -186,11 +201,13 @@ void seg_dims(FitSeg *d_segs, int B, const ChainRule &rule, int d, hipStream_t s
 void seg_init_embedding(const FitSeg *d_segs, int B, const SegTotals &tot, int d, int n_init_epochs, uint64_t seed,
                         hipStream_t st);
 // All SGD epochs of every data set from Q0 (E, T, mode must be set); no host synchronisation.
-// epoch_events (optional, 2 * n_epochs events) bracket each epoch kernel.
+// Each epoch runs as sgd_sub_epochs(tot) launches of the epoch kernel (plus the copies' merges);
+// epoch_events (optional, 2 * n_epochs * sgd_sub_epochs(tot) events) bracket each launch.
+int sgd_sub_epochs(const SegTotals &tot);
 void seg_optimize(const FitSeg *d_segs, int B, const SegTotals &tot, int d, float Q0, const SgdParams &p,
                   hipStream_t st, hipEvent_t *epoch_events);
-// summed time of the epoch kernels between event pairs
-double epoch_time_ms(const hipEvent_t *ev, int n_epochs);
+// summed time of the epoch kernel launches between event pairs
+double epoch_time_ms(const hipEvent_t *ev, int n_launches);
 
 // One data set, synchronous (the stage entry points): random layout + propagation epochs with
 // `chains` threads' static schedule of the random draws.
@@ -198,13 +215,14 @@ void init_embedding_run(int64_t N, int d, const int64_t *d_row_ptr, const int32_
                         int n_init_epochs, uint64_t seed, int64_t chains, float *dY, hipStream_t st);
 
 // Device scratch of the epoch loop, for callers that bring their own memory: chain_row [T],
-// rng [T] (16-byte aligned), qpart [T], qdev [1 + n_epochs], seg [1].
+// rng [T] (16-byte aligned), qpart [T], qdev [1 + n_epochs], seg [1], sub_row [T].
 struct SgdWorkspace {
     int64_t *chain_row;
     U128 *rng;
     float *qpart;
     float *qdev;
     FitSeg *seg;
+    int64_t *sub_row;  // [T]
 };
 
 // One data set with host-known E and chain count, synchronous; Q (host in/out). `q_trace`

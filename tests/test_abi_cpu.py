@@ -147,3 +147,13 @@ def test_tune_cache_reader(tmp_path):
             abi.load_tune_cache(tmp_path / "missing.json")
     finally:
         assert abi.load_tune_cache(None) == 0
+
+
+def test_sgd_copy_knobs_are_validated():
+    # the per-XCD copy knobs are range-checked without touching a GPU
+    from ncvis_amd import abi
+    for key, bad in (("sgd_replicas", 2), ("sgd_replicas", -1), ("sgd_sync", 0), ("sgd_sync", 65)):
+        with pytest.raises(RuntimeError):
+            abi.tune(key, bad)
+    for key, good in (("sgd_replicas", 1), ("sgd_sync", 16), ("sgd_edge_flush", 1)):
+        abi.tune(key, good)

@@ -303,12 +303,13 @@ def test_optimize_single_chain_d1_and_d3(torch_cuda):
 
 
 @pytest.mark.parametrize("T", [3, 64, 4096])
-@pytest.mark.parametrize("atomic", [0, 1])
+@pytest.mark.parametrize("atomic", [0, 1, 2])
 def test_optimize_chains_alpha0_q_trace(torch_cuda, T, atomic):
     # alpha = 0: every step is 0 (ncvis.cpp:287), Y never changes and each chain's Q copy depends
     # only on its static edge chunk, its pcg64(seed + t) stream (kept across epochs) and the
     # epoch's Q, so the chain partition, the seeding and the per-epoch Q average (:342-349) are
-    # deterministic. atomic = 1 at d = 2 is the default many-chain epoch kernel.
+    # deterministic. atomic = 1 at d = 2 is the many-chain epoch on device atomics; atomic = 2 the
+    # epoch on per-XCD copies of Y, split into sub-epochs with the copies merged after each.
     _say(f"optimize alpha=0 T={T} atomic={atomic}")
     X, inds, dists, row_ptr, col = _digits_graph(N=2000, D=32, k=15)
     N, k = inds.shape
@@ -332,13 +333,56 @@ def test_optimize_chains_hogwild_q_within_reference_scatter(torch_cuda):
     plan = [3] * 10
     oq = np.array([oracle.optimize(Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 1.0, 1.0, s, 8)[1]
                    for s in (42, 43, 44, 45)])
-    for atomic, T in ((0, 64), (1, 64), (1, 0)):
+    for atomic, T in ((0, 64), (1, 64), (1, 0), (2, 64), (2, 0)):
         gy, gq, _ = gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 1.0, 1.0,
                                  42, T, atomic)
         print(f"atomic={atomic} T={T}: gpu Q {gq:.4f}; oracle Q over seeds {oq}", flush=True)
         assert np.all(np.isfinite(gy))
         tol = 3 * max(float(oq.std()), 0.01 * abs(float(oq.mean())))
         assert abs(gq - oq.mean()) <= tol, (gq, oq)
+
+
+def test_sub_epochs_keep_the_single_chain_fit(torch_cuda):
+    # the epochs run as sgd_sync consecutive launches that carry every chain's pcg64 state, Q copy
+    # and current row over; one exact chain must give the same bits however the epoch is split
+    from ncvis_amd import abi
+    torch = torch_cuda
+    X, _ = synthetic.digits_like(N=700, D=32, seed=4)
+    dX = _dev(torch, X)
+    out = []
+    try:
+        for S in (1, 3, 7):
+            _say(f"single chain, {S} sub-epochs")
+            abi.tune("sgd_sync", S)
+            h = abi.Handle(d=2, n_threads=1, n_neighbors=15, n_noise=[2, 3, 1], a=A_DEF, b=B_DEF)
+            h.set_option("exact_threads", 1)
+            dY = torch.empty((700, 2), dtype=torch.float32, device="cuda")
+            h.fit_transform_device(dX.data_ptr(), 700, 32, dY.data_ptr(), _stream(torch))
+            torch.cuda.synchronize()
+            out.append((dY.cpu().numpy(), h.stats()["Q"]))
+    finally:
+        abi.tune("sgd_sync", 16)
+    for Y, q in out[1:]:
+        assert np.array_equal(Y, out[0][0]) and q == out[0][1]
+
+
+def test_replica_option_validation(torch_cuda):
+    from ncvis_amd import abi
+    h = abi.Handle(d=2, n_threads=0, n_neighbors=10, n_noise=[1])
+    for bad in (-2, 3):
+        with pytest.raises(RuntimeError, match="atomic_updates"):
+            h.set_option("atomic_updates", bad)
+    X, inds, dists, row_ptr, col = _digits_graph(N=300)
+    Y0 = np.zeros((300, 5), np.float32)
+    with pytest.raises(RuntimeError, match="d <= 4"):
+        gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, [1], A_DEF, B_DEF, 1.0, 1.0, 42, 8, 2)
+    with pytest.raises(RuntimeError, match="allocates"):
+        from ncvis_amd import abi as _a
+        rp = _dev(torch_cuda, row_ptr.astype(np.int64))
+        cl = _dev(torch_cuda, col.astype(np.int32))
+        Y = _dev(torch_cuda, np.zeros((300, 2), np.float32))
+        _a.optimize(300, 2, rp.data_ptr(), cl.data_ptr(), len(col), Y.data_ptr(), 0.0, [1], A_DEF, B_DEF, 1.0,
+                    1.0, 42, 8, 2, _stream(torch_cuda), noalloc=True)
 
 
 # ------------------------------------------------------------------------------ NCE loss

@@ -133,16 +133,20 @@ def profile_loss(Y, src, dst, neg, a, b):
     return oracle.nce_loss(Y, q, src, dst, neg, a, b, grad=False), q
 
 
-def test_full_run_parity_many_chains(ncvis):
+@pytest.mark.parametrize("mode", ["default", "xcd_copies"])
+def test_full_run_parity_many_chains(ncvis, mode):
     """digits-shaped data (N=5,620, D=64, integers, config 1's shape): the default GPU fit vs
     the oracle run with 8 OpenMP threads (default plan, k=15) at seeds 42, 43, 44. Criteria:
     15-NN preservation within 0.01 of the oracle's seed band, and the NCE loss at each run's
     own final Q (negatives: a fixed table of the last epoch's count per edge, the count that
     final Q is learned against) within 2% of the oracle's seed band; the loss minimised over Q
-    is held to the same bound."""
-    _say("full-run parity digits 5620x64")
+    is held to the same bound. mode xcd_copies: the epochs forced onto per-XCD copies of Y
+    (atomic_updates 2), which this size (about 2.4 rows per chain) would not pick by itself."""
+    _say(f"full-run parity digits 5620x64 {mode}")
     X, _ = synthetic.digits_like(seed=0)
     vis = ncvis.NCVis(n_threads=-1, random_seed=42)
+    if mode == "xcd_copies":
+        vis.model.set_option("atomic_updates", 2)
     Y = vis.fit_transform(X)
     Qg = vis.stats()["Q"]
     plan = vis.negative_plan
