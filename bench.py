@@ -79,8 +79,10 @@ def cpu_model():
 
 
 def measured_traffic(workload_desc, precision):
-    """HBM bytes per launch of the two dominant kernels, from the calibrated FETCH_SIZE/WRITE_SIZE
-    counter runs committed for the 1M x 128 workload; {} for other workloads or precisions."""
+    """HBM bytes per launch of the dominant kernels (by role: knn_search = the symmetric search's
+    three kernels, sgd_epoch = one launch of the copy-mode epoch kernel), from the calibrated
+    FETCH_SIZE/WRITE_SIZE counter runs committed for the 1M x 128 workload; {} for other
+    workloads or precisions."""
     path = os.path.join(ROOT, "profiles", "traffic_fit_1m_d128.json")
     try:
         with open(path) as f:
@@ -330,7 +332,7 @@ def main():
     traffic = measured_traffic(w["desc"], args.precision)
     roof_knn = {"bound": "mfma", "kernel": "k_knn_tiles", "achieved": knn_flops / knn_tiles_s / 1e12,
                 "peak": MFMA_PEAK_TFLOPS[args.precision], "unit": "TFLOP/s",
-                "traffic": traffic.get("k_knn_tiles"), "launch_ms": stage["knn_tiles"]}
+                "traffic": None, "launch_ms": stage["knn_tiles"]}
     roof_knn["frac"] = roof_knn["achieved"] / roof_knn["peak"]
     # the MFMA work the chosen search variant executes: the symmetric search scores each pair of
     # 256-row blocks once (diagonal blocks in full) plus its bounding pass over every sym-th row
@@ -339,13 +341,15 @@ def main():
         nbI = -(-N // 256)
         executed = 2.0 * D * 256 * 256 * nbI * (nbI + 1) / 2 + 2.0 * D * N * (-(-N // kplan["sym"]))
         roof_knn["kernel"] = "k_sym_bound + k_sym_main + k_sym_select"
+        roof_knn["traffic"] = traffic.get("knn_search")
     else:
         executed = knn_flops
     roof_knn["variant"] = kplan
     roof_knn["executed_flops"] = executed
     roof_knn["executed_frac"] = executed / knn_tiles_s / 1e12 / roof_knn["peak"]
     roof_sgd = {"bound": "hbm", "kernel": "k_sgd_epoch", "achieved": sgd_bytes / sgd_s / 1e9,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": traffic.get("k_sgd_epoch"),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "traffic": traffic.get("sgd_epoch") if s_last["sgd_mode"] == 3 else None,
                 "launch_ms": stage["sgd_kernel"] / s_last["sgd_launches"], "launches": s_last["sgd_launches"]}
     roof_sgd["frac"] = roof_sgd["achieved"] / roof_sgd["peak"]
     # the epoch's real bound: one random 8-byte-row read + update per sample, calibrated on a

@@ -19,7 +19,8 @@ struct Tuning {
     int sgd_sync = 16;               // sub-epochs per epoch: every chain writes its source row out and,
                                      // with copies, the copies are merged after each
     int sgd_edge_flush = 1;          // per-XCD copies: write the source row's update after every edge
-    int sgd_fast_math = 0;           // 1: single-precision powf/expf in the many-chain epoch
+    int sgd_fast_math = 1;           // many-chain epochs: single-precision powf/expf (0: double, rounded
+                                     // once, as the exact one-chain mode always does)
     int64_t edges_per_chain = 64;    // auto chain count = E / edges_per_chain,
     int64_t min_chains = 1;          // ... at least min(E / 16, min_chains) (1: off) ...
     int64_t max_chains = 1 << 17;    // ... and at most this

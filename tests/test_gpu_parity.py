@@ -471,7 +471,8 @@ def test_knn_tuning_knobs_keep_the_table(torch_cuda, knobs, precision):
     assert np.array_equal(gi, oi) and np.array_equal(gd, od)
 
 
-@pytest.mark.parametrize("knobs", [{"sgd_prefetch": 4}, {"sgd_fast_math": 1}, {"max_chains": 64}])
+@pytest.mark.parametrize("knobs", [{"sgd_prefetch": 4}, {"sgd_fast_math": 0}, {"max_chains": 64},
+                                   {"sgd_edge_flush": 0}, {"sgd_sync": 1}, {"sgd_replicas": 0}])
 def test_sgd_tuning_knobs_run(torch_cuda, knobs):
     # the many-chain epoch variants the SGD knobs select all descend the same objective
     from ncvis_amd import abi
@@ -479,16 +480,21 @@ def test_sgd_tuning_knobs_run(torch_cuda, knobs):
     X, inds, dists, row_ptr, col = _digits_graph(N=1500, D=32, k=15, seed=5)
     N = X.shape[0]
     Y0 = oracle.init_embedding(N, 2, row_ptr, col, np.float32(1.0 / 15), 5, 42, 1)
-    defaults = {"sgd_prefetch": 1, "sgd_fast_math": 0, "max_chains": 1 << 17}
+    defaults = {"sgd_prefetch": 1, "sgd_fast_math": 1, "max_chains": 1 << 17, "sgd_edge_flush": 1, "sgd_sync": 16,
+                "sgd_replicas": 1}
+    runs = []
     try:
         for key, v in knobs.items():
             abi.tune(key, v)
-        Y, q, _ = gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, [3] * 20, A_DEF, B_DEF, 1.0, 1.0, 42, 0, 1)
+        for atomic in (1, 2):  # device atomics, per-XCD copies
+            runs.append(gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, [3] * 20, A_DEF, B_DEF, 1.0, 1.0, 42, 0,
+                                     atomic))
     finally:
         for key, v in defaults.items():
             abi.tune(key, v)
-    assert np.all(np.isfinite(Y)) and np.isfinite(q)
-    assert Y.std() > 0.5 * Y0.std()
+    for Y, q, _ in runs:
+        assert np.all(np.isfinite(Y)) and np.isfinite(q)
+        assert Y.std() > 0.5 * Y0.std()
 
 
 def test_knn_autotune_cache_round_trip(torch_cuda, tmp_path):
