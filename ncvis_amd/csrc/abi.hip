@@ -664,6 +664,9 @@ int ncvis_tune(const char *key, double value) {
                 throw std::invalid_argument("[ncvis_amd] knn_sym: -1 (by size), 0 (list tiles) or a bounding-row "
                                             "stride 1..64");
             t.knn_sym = (int)value;
+        } else if (k == "knn_sym_chunk") {
+            if (value < 0 || value > 4096) throw std::invalid_argument("[ncvis_amd] knn_sym_chunk: 0..4096 stages");
+            t.knn_sym_chunk = (int)value;
         } else if (k == "knn_stage_mult") {
             if (value != 1 && value != 2) throw std::invalid_argument("[ncvis_amd] knn_stage_mult: 1 or 2");
             t.knn_stage_mult = (int)value;

@@ -29,6 +29,8 @@ struct Tuning {
     int knn_split = 0;               // candidate split of untuned searches (0: by workgroup count)
     int knn_sym = -1;                // 16-bit kNN, k + 1 <= 16: symmetric threshold search with row
                                      // bounds from every knn_sym-th row (0: list tiles; -1: by size)
+    int knn_sym_chunk = 128;         // symmetric search: candidate stages per chunk of the chunk-major
+                                     // work order (0: each workgroup sweeps two query blocks' ranges)
 };
 Tuning &tuning();
 

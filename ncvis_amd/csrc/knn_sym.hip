@@ -307,11 +307,22 @@ __device__ __forceinline__ void sym_dma_stage(const unsigned char *__restrict__ 
     }
 }
 
+// Work items of the chunk-major order: candidate chunk c (stages [c * chunk, (c + 1) * chunk))
+// against every query block I whose own stages start before the chunk ends, I < (c + 1) * m with
+// m = chunk / SPB query blocks per chunk; items run chunk by chunk, so the workgroups resident on
+// an XCD at one time sweep the same chunk (chunk * CB rows, which the XCD's L2 holds) instead of
+// each streaming its own range of the whole candidate matrix from memory.
+__host__ __device__ inline int64_t sym_chunk_prefix(int64_t c, int64_t m, int64_t nbI) {
+    const int64_t full = nbI / m;            // chunks c with (c + 1) m <= nbI
+    const int64_t k = c < full ? c : full;   // of the first c chunks, those below the cap
+    return m * k * (k + 1) / 2 + (c - k) * nbI;
+}
+
 template <int P, int DK>
 __global__ __launch_bounds__(256, 2) void k_sym_main(const void *__restrict__ Xh_, const float *__restrict__ cn,
                                                      const float *__restrict__ Tq, int64_t N, int64_t Npad,
                                                      float beta, float sigma, int B, int *__restrict__ cnt,
-                                                     uint2 *__restrict__ bucket) {
+                                                     uint2 *__restrict__ bucket, int chunk) {
     using D = SymDma<DK>;
     constexpr int CB = D::CB, SPB = D::SPB, KS = DK * 2;
     __shared__ __align__(16) unsigned char lds[2][CB * D::ROWB];
@@ -326,17 +337,41 @@ __global__ __launch_bounds__(256, 2) void k_sym_main(const void *__restrict__ Xh
     const int col = lane & 31;
     const int swz = col & (D::CHUNKS - 1);  // rows rb * 32 + col share col's swizzle
 
-    // the two query blocks of this workgroup and its share of their concatenated sweeps
     const int64_t nbI = Npad / 256, NS = Npad / CB;
-    const int64_t I1 = blockIdx.x, I2 = nbI - 1 - (int64_t)blockIdx.x;
-    const int64_t len1 = NS - I1 * SPB, len2 = (I2 > I1) ? NS - I2 * SPB : 0;
-    const int64_t Lt = len1 + len2;
-    const int64_t vb = Lt * blockIdx.y / gridDim.y, ve = Lt * (blockIdx.y + 1) / gridDim.y;
+    int64_t I1, I2, len1, vb, ve, s_lo = 0;
+    if (chunk > 0) {
+        // chunk-major item: chunk c, query block I1, stages [max(c chunk, I1 SPB), min((c+1) chunk, NS))
+        const int64_t m = chunk / SPB, nC = (NS + chunk - 1) / chunk, item = blockIdx.x;
+        int64_t lo = 0, hi = nC;  // largest c with prefix(c) <= item
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (sym_chunk_prefix(mid, m, nbI) <= item)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        I1 = item - sym_chunk_prefix(lo, m, nbI);
+        I2 = I1;
+        s_lo = lo * chunk > I1 * SPB ? lo * chunk : I1 * SPB;
+        const int64_t s_hi = (lo + 1) * chunk < NS ? (lo + 1) * chunk : NS;
+        len1 = s_hi - s_lo;  // virtual stage v = stage s_lo + v, all of block I1
+        vb = 0;
+        ve = len1;
+    } else {
+        // the two query blocks of this workgroup and its share of their concatenated sweeps
+        I1 = blockIdx.x;
+        I2 = nbI - 1 - (int64_t)blockIdx.x;
+        len1 = NS - I1 * SPB;
+        const int64_t len2 = (I2 > I1) ? NS - I2 * SPB : 0;
+        const int64_t Lt = len1 + len2;
+        vb = Lt * blockIdx.y / gridDim.y;
+        ve = Lt * (blockIdx.y + 1) / gridDim.y;
+    }
     if (vb >= ve) return;  // uniform over the workgroup
-    // virtual stage v: block I1's stages from the last one down to its own rows, then block I2's:
-    // every workgroup sweeps the candidate stages in the same (descending) order, so workgroups
-    // running together read the same stages and share them in L2
-#define NCVIS_SYM_STAGE(V) ((V) < len1 ? NS - 1 - (V) : NS - 1 - ((V)-len1))
+    // pair order: virtual stage v is block I1's stages from the last one down to its own rows, then
+    // block I2's (every workgroup sweeps the candidate stages in the same descending order);
+    // chunk order: the item's stages ascending
+#define NCVIS_SYM_STAGE(V) (chunk > 0 ? s_lo + (V) : ((V) < len1 ? NS - 1 - (V) : NS - 1 - ((V)-len1)))
 
     uint4 qf[2][KS];
     int qi[2] = {0, 0};
@@ -640,8 +675,17 @@ static void sym_launch(const void *Xh, const float *cn, int64_t N, int64_t Npad,
         k_sym_bound<P, DK, 8><<<(unsigned)(Npad / 256), 256, 0, st>>>(Xh, cn, N, beta, stride, Ns, sigma, w.T);
     NCVIS_LAUNCH_CHECK();
     const int64_t nbI = Npad / 256;
-    k_sym_main<P, DK><<<dim3((unsigned)((nbI + 1) / 2), (unsigned)split), 256, 0, st>>>(
-        Xh, cn, w.T, N, Npad, beta, sigma, B, w.cnt, w.bucket);
+    const int NS = (int)(Npad / SymDma<DK>::CB), SPB = SymDma<DK>::SPB;
+    int chunk = tuning().knn_sym_chunk;
+    if (chunk > 0) chunk = std::max(SPB, chunk / SPB * SPB);  // whole query blocks per chunk
+    if (chunk > 0) {
+        const int64_t items = sym_chunk_prefix((NS + chunk - 1) / chunk, chunk / SPB, nbI);
+        k_sym_main<P, DK><<<dim3((unsigned)items, 1u), 256, 0, st>>>(Xh, cn, w.T, N, Npad, beta, sigma, B, w.cnt,
+                                                                     w.bucket, chunk);
+    } else {
+        k_sym_main<P, DK><<<dim3((unsigned)((nbI + 1) / 2), (unsigned)split), 256, 0, st>>>(
+            Xh, cn, w.T, N, Npad, beta, sigma, B, w.cnt, w.bucket, 0);
+    }
     NCVIS_LAUNCH_CHECK();
     int *n_ovf = w.cnt + Npad;
     k_sym_select<<<cdiv(N, 4), 256, 0, st>>>(w.cnt, w.bucket, B, N, K1, cand, w.ovf, n_ovf);

@@ -28,9 +28,10 @@ def torch_cuda():
     return torch
 
 
-def _knn(torch, X, k, distance, precision, sym):
+def _knn(torch, X, k, distance, precision, sym, chunk=128):
     """GPU table with the symmetric search forced on with row bounds from every sym-th row (0: the
-    list tiles)."""
+    list tiles); chunk: candidate stages per chunk of the emission pass's work order (0: each
+    workgroup sweeps a pair of query blocks)."""
     from ncvis_amd import abi
     N, D = X.shape
     dX = torch.from_numpy(np.ascontiguousarray(X.astype(np.float32))).cuda()
@@ -38,11 +39,13 @@ def _knn(torch, X, k, distance, precision, sym):
     dists = torch.empty((N, k), dtype=torch.float32, device="cuda")
     try:
         abi.tune("knn_sym", sym)
+        abi.tune("knn_sym_chunk", chunk)
         abi.knn(dX.data_ptr(), N, D, k, distance, precision, inds.data_ptr(), dists.data_ptr(),
                 torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
     finally:
         abi.tune("knn_sym", -1)
+        abi.tune("knn_sym_chunk", 128)
     return inds.cpu().numpy().astype(np.int64), dists.cpu().numpy()
 
 
@@ -56,6 +59,18 @@ def test_sym_exact_on_integer_data(torch_cuda, D, precision, N, stride):
     X, _ = synthetic.digits_like(N=N, D=D, seed=31 + D)
     oi, od = oracle.knn_bruteforce(oracle.preprocess(X), 15)
     gi, gd = _knn(torch_cuda, X, 15, "euclidean", precision, stride)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+
+
+# the emission pass's work order: pairs of query blocks per workgroup (0), chunk-major items with
+# chunks of 2 query blocks (4 stages: many items, a partial last chunk) or 3 (6), and the default
+@pytest.mark.parametrize("chunk", [0, 4, 6, 128])
+@pytest.mark.parametrize("D", [20, 128])
+def test_sym_work_orders_give_the_same_table(torch_cuda, chunk, D):
+    _say(f"sym knn chunk={chunk} D={D}")
+    X, _ = synthetic.digits_like(N=5000, D=D, seed=41 + D)
+    oi, od = oracle.knn_bruteforce(oracle.preprocess(X), 15)
+    gi, gd = _knn(torch_cuda, X, 15, "euclidean", "fp16", 4, chunk)
     assert np.array_equal(gi, oi) and np.array_equal(gd, od)
 
 
