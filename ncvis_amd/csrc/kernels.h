@@ -16,7 +16,7 @@ struct Tuning {
     int sgd_plain_far = 0;           // 1: far-endpoint updates with plain stores instead of atomics
     int sgd_replicas = 1;            // many-chain epochs on per-XCD copies of Y (mode 3): 1 when the
                                      // atomics are automatic, 0 never (device atomics, mode 2)
-    int sgd_sync = 16;               // sub-epochs per epoch: every chain writes its source row out and,
+    int sgd_sync = 20;               // sub-epochs per epoch: every chain writes its source row out and,
                                      // with copies, the copies are merged after each
     int sgd_edge_flush = 1;          // per-XCD copies: write the source row's update after every edge
     int sgd_fast_math = 1;           // many-chain epochs: single-precision powf/expf (0: double, rounded

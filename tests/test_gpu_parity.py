@@ -342,6 +342,25 @@ def test_optimize_chains_hogwild_q_within_reference_scatter(torch_cuda):
         assert abs(gq - oq.mean()) <= tol, (gq, oq)
 
 
+@pytest.mark.parametrize("d", [1, 3, 4])
+def test_xcd_copies_every_embedding_width(torch_cuda, d):
+    # the per-XCD-copy epochs read and write rows as coordinate pairs (even d) or single floats
+    # (odd d). alpha = 0: Y stays, the merges keep it bit for bit, and the Q trace is the oracle's;
+    # alpha = 1: a finite embedding that moves
+    _say(f"xcd copies d={d}")
+    X, inds, dists, row_ptr, col = _digits_graph(N=2000, D=32, k=15)
+    N, k = inds.shape
+    Y0 = oracle.init_embedding(N, d, row_ptr, col, np.float32(1.0 / k), 5, 42, 1)
+    plan = [3, 1, 2]
+    gy, gq, gtrace = gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 0.0, 1.0, 42, 64, 2)
+    _, oq, otrace = oracle.optimize_chains(Y0, 0.0, row_ptr, col, plan, A_DEF, B_DEF, 0.0, 1.0, 42, 64)
+    assert np.array_equal(gy, Y0)
+    np.testing.assert_allclose(gtrace, otrace, rtol=1e-5, atol=1e-5)
+    gy, gq, _ = gpu_optimize(torch_cuda, Y0, 0.0, row_ptr, col, [3] * 10, A_DEF, B_DEF, 1.0, 1.0, 42, 0, 2)
+    assert np.all(np.isfinite(gy)) and np.isfinite(gq)
+    assert np.abs(gy - Y0).max() > 0.1 and gy.std() > 0.5 * Y0.std()
+
+
 def test_sub_epochs_keep_the_single_chain_fit(torch_cuda):
     # the epochs run as sgd_sync consecutive launches that carry every chain's pcg64 state, Q copy
     # and current row over; one exact chain must give the same bits however the epoch is split
@@ -361,7 +380,7 @@ def test_sub_epochs_keep_the_single_chain_fit(torch_cuda):
             torch.cuda.synchronize()
             out.append((dY.cpu().numpy(), h.stats()["Q"]))
     finally:
-        abi.tune("sgd_sync", 16)
+        abi.tune("sgd_sync", 20)
     for Y, q in out[1:]:
         assert np.array_equal(Y, out[0][0]) and q == out[0][1]
 
@@ -480,7 +499,7 @@ def test_sgd_tuning_knobs_run(torch_cuda, knobs):
     X, inds, dists, row_ptr, col = _digits_graph(N=1500, D=32, k=15, seed=5)
     N = X.shape[0]
     Y0 = oracle.init_embedding(N, 2, row_ptr, col, np.float32(1.0 / 15), 5, 42, 1)
-    defaults = {"sgd_prefetch": 1, "sgd_fast_math": 1, "max_chains": 1 << 17, "sgd_edge_flush": 1, "sgd_sync": 16,
+    defaults = {"sgd_prefetch": 1, "sgd_fast_math": 1, "max_chains": 1 << 17, "sgd_edge_flush": 1, "sgd_sync": 20,
                 "sgd_replicas": 1}
     runs = []
     try:
