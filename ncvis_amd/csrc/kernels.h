@@ -18,7 +18,8 @@ struct Tuning {
                                      // atomics are automatic, 0 never (device atomics, mode 2)
     int sgd_sync = 20;               // sub-epochs per epoch: every chain writes its source row out and,
                                      // with copies, the copies are merged after each
-    int sgd_edge_flush = 1;          // per-XCD copies: write the source row's update after every edge
+    int sgd_edge_flush = 0;          // per-XCD copies: write the source row's update after every edge
+                                     // (0: at row changes and sub-epoch ends)
     int sgd_fast_math = 1;           // many-chain epochs: single-precision powf/expf (0: double, rounded
                                      // once, as the exact one-chain mode always does)
     int64_t edges_per_chain = 64;    // auto chain count = E / edges_per_chain,
