@@ -166,6 +166,9 @@ __global__ __launch_bounds__(256) void k_pack_params(const double *__restrict__ 
     }
 }
 
+// rows per k_pack block: LDS holds R rows of Dpad + 1 floats (at most 33 KB)
+inline int pack_rows(int Dpad) { return Dpad <= 256 ? 32 : 16; }
+
 // Element k of a padded row as stored for MFMA: (x_k - c_k) * s, rounded to the GEMM precision.
 // 16-bit precisions keep the natural order (lane half h of a 32x32x16 step reads
 // k = 16 s + 8 h .. +7, contiguous). FP32 is stored permuted so a lane's operands are contiguous
@@ -173,40 +176,55 @@ __global__ __launch_bounds__(256) void k_pack_params(const double *__restrict__ 
 // 16x16x4: G = 4, group g reads k = 4 s + g) element k = G s + g sits at position g (Dpad / G) + s.
 // A and B use the same permutation, so dot products are unchanged.
 template <int P, int G = 2>
-__global__ void k_pack(const float *__restrict__ Xp, int64_t N, int64_t Npad, int D, int Dpad, int is_l2,
-                       const float *__restrict__ cs, void *__restrict__ Xh, float *__restrict__ cn) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Npad) return;
+__global__ __launch_bounds__(256) void k_pack(const float *__restrict__ Xp, int64_t N, int64_t Npad, int D, int Dpad,
+                                              int is_l2, const float *__restrict__ cs, void *__restrict__ Xh,
+                                              float *__restrict__ cn, int R) {
+    // R rows per block: the 256 threads read the rows' elements coalesced, write the
+    // rounded elements and keep them in LDS in stored order; one thread per row then sums their
+    // squares sequentially in stored order (the score offset cn)
+    extern __shared__ float lds_pack[];  // [R][Dpad + 1]
+    const int64_t r0 = (int64_t)blockIdx.x * R;
     const float sc = cs[D];
-    float s = 0.f;
-    if (P == PREC_FP32) {
-        float *o = (float *)Xh + i * Dpad;
-        const int part = Dpad / G;
-        for (int p = 0; p < Dpad; ++p) {
-            int h = p / part, st = p - h * part;
-            int kk = G * st + h;
-            float v = (i < N && kk < D) ? __fmul_rn(__fsub_rn(Xp[i * D + kk], cs[kk]), sc) : 0.f;
-            o[p] = v;
-            s = __fadd_rn(s, __fmul_rn(v, v));
-        }
-    } else {
-        uint16_t *o = (uint16_t *)Xh + i * Dpad;
-        for (int kk = 0; kk < Dpad; ++kk) {
-            float v = (i < N && kk < D) ? __fmul_rn(__fsub_rn(Xp[i * D + kk], cs[kk]), sc) : 0.f;
-            float r;
+    const int part = Dpad / G;
+    const int pitch = Dpad + 1;
+    for (int e = threadIdx.x; e < R * Dpad; e += 256) {
+        const int row = e / Dpad, kk = e - row * Dpad;  // kk: element index (natural order)
+        const int64_t i = r0 + row;
+        if (i >= Npad) continue;
+        const float v = (i < N && kk < D) ? __fmul_rn(__fsub_rn(Xp[i * D + kk], cs[kk]), sc) : 0.f;
+        float r;
+        int pos;
+        if (P == PREC_FP32) {
+            pos = (kk % G) * part + kk / G;  // element G st + g sits at position g (Dpad / G) + st
+            ((float *)Xh)[i * Dpad + pos] = v;
+            r = v;
+        } else {
+            pos = kk;
+            uint16_t bits;
             if (P == PREC_BF16) {
-                uint16_t b = f32_to_bf16_rne(v);
-                o[kk] = b;
-                r = bf16_to_f32(b);
+                bits = f32_to_bf16_rne(v);
+                r = bf16_to_f32(bits);
             } else {
-                _Float16 hv = (_Float16)v;
-                o[kk] = __builtin_bit_cast(uint16_t, hv);
+                const _Float16 hv = (_Float16)v;
+                bits = __builtin_bit_cast(uint16_t, hv);
                 r = (float)hv;
             }
-            s = __fadd_rn(s, __fmul_rn(r, r));
+            ((uint16_t *)Xh)[i * Dpad + pos] = bits;
+        }
+        lds_pack[row * pitch + pos] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < R) {
+        const int64_t i = r0 + threadIdx.x;
+        if (i < Npad) {
+            float s = 0.f;
+            for (int p = 0; p < Dpad; ++p) {
+                const float r = lds_pack[threadIdx.x * pitch + p];
+                s = __fadd_rn(s, __fmul_rn(r, r));
+            }
+            cn[i] = (i < N) ? (is_l2 ? s : 0.f) : INFINITY;
         }
     }
-    cn[i] = (i < N) ? (is_l2 ? s : 0.f) : INFINITY;
 }
 
 // ------------------------------------------------------------------------ exact distances
@@ -228,33 +246,45 @@ __device__ inline bool key_less(float da, int ia, int la, float db, int ib, int 
 }
 
 // Exact distance with row loads four floats at a time; the sum stays sequential and uncontracted
-// (same value as exact_dist). Rows must be 16-byte aligned (D % 4 == 0 and an aligned base).
-__device__ inline float exact_dist4(const float *__restrict__ x, const float *__restrict__ y, int D, int is_l2) {
-    const float4 *x4 = (const float4 *)x;
-    const float4 *y4 = (const float4 *)y;
-    float s = 0.f;
+// (same value as exact_dist). Rows must be 16-byte aligned (D % 4 == 0 and an aligned base). The
+// loads go out a 128-byte line at a time (8 float4 of each row) before their terms are summed, so
+// a lane's requests for one line arrive together instead of one per step of the sum.
+__device__ inline void dist4_terms(float &s, float4 a, float4 b, int is_l2) {
     if (is_l2) {
-        for (int j = 0; j < D / 4; ++j) {
-            const float4 a = x4[j], b = y4[j];
-            float t = __fsub_rn(a.x, b.x);
-            s = __fadd_rn(s, __fmul_rn(t, t));
-            t = __fsub_rn(a.y, b.y);
-            s = __fadd_rn(s, __fmul_rn(t, t));
-            t = __fsub_rn(a.z, b.z);
-            s = __fadd_rn(s, __fmul_rn(t, t));
-            t = __fsub_rn(a.w, b.w);
-            s = __fadd_rn(s, __fmul_rn(t, t));
-        }
-        return s;
-    }
-    for (int j = 0; j < D / 4; ++j) {
-        const float4 a = x4[j], b = y4[j];
+        float t = __fsub_rn(a.x, b.x);
+        s = __fadd_rn(s, __fmul_rn(t, t));
+        t = __fsub_rn(a.y, b.y);
+        s = __fadd_rn(s, __fmul_rn(t, t));
+        t = __fsub_rn(a.z, b.z);
+        s = __fadd_rn(s, __fmul_rn(t, t));
+        t = __fsub_rn(a.w, b.w);
+        s = __fadd_rn(s, __fmul_rn(t, t));
+    } else {
         s = __fadd_rn(s, __fmul_rn(a.x, b.x));
         s = __fadd_rn(s, __fmul_rn(a.y, b.y));
         s = __fadd_rn(s, __fmul_rn(a.z, b.z));
         s = __fadd_rn(s, __fmul_rn(a.w, b.w));
     }
-    return __fsub_rn(1.0f, s);
+}
+
+__device__ inline float exact_dist4(const float *__restrict__ x, const float *__restrict__ y, int D, int is_l2) {
+    const float4 *x4 = (const float4 *)x;
+    const float4 *y4 = (const float4 *)y;
+    const int n4 = D / 4;
+    float s = 0.f;
+    int j = 0;
+    for (; j + 8 <= n4; j += 8) {
+        float4 a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a[u] = x4[j + u];
+            b[u] = y4[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) dist4_terms(s, a[u], b[u], is_l2);
+    }
+    for (; j < n4; ++j) dist4_terms(s, x4[j], y4[j], is_l2);
+    return is_l2 ? s : __fsub_rn(1.0f, s);
 }
 
 // C = 2*KL candidates per query (32 or 64); a wave reranks 64/C queries, lane c of a query's
@@ -631,18 +661,21 @@ void knn_enqueue(const float *dX, int64_t N, int D, int k, int dist, int prec, i
         float *cs = (float *)(w + L.cs);
         int32_t *cand = (int32_t *)(w + L.cand);
         gemm_input_transform(Xp, N, D, is_l2, L.nb, (double *)(w + L.part), (double *)(w + L.mean), cs, st);
+        const int pr = pack_rows(Dpad);
+        const unsigned pb = (unsigned)cdiv(Npad, pr);
+        const size_t plds = sizeof(float) * pr * (Dpad + 1);
         switch (prec) {
             case PREC_FP32:
                 if (L.wide32)
-                    k_pack<PREC_FP32, 4><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs, xh, cn);
+                    k_pack<PREC_FP32, 4><<<pb, 256, plds, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs, xh, cn, pr);
                 else
-                    k_pack<PREC_FP32, 2><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs, xh, cn);
+                    k_pack<PREC_FP32, 2><<<pb, 256, plds, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs, xh, cn, pr);
                 break;
             case PREC_BF16:
-                k_pack<PREC_BF16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs, xh, cn);
+                k_pack<PREC_BF16><<<pb, 256, plds, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs, xh, cn, pr);
                 break;
             default:
-                k_pack<PREC_FP16><<<cdiv(Npad, 256), 256, 0, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs, xh, cn);
+                k_pack<PREC_FP16><<<pb, 256, plds, st>>>(Xp, N, Npad, D, Dpad, is_l2, cs, xh, cn, pr);
                 break;
         }
         NCVIS_LAUNCH_CHECK();
