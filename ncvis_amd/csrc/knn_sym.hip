@@ -190,18 +190,28 @@ __global__ __launch_bounds__(256, 2) void k_sym_bound(const void *__restrict__ X
             }
 #pragma unroll
             for (int qb = 0; qb < 2; ++qb) {
-                float m = INFINITY;
+                // minima of the lane's four groups of four scores: the insertion walks only the
+                // groups in which some lane beats its (falling) threshold, in the same order
+                float m[4];
 #pragma unroll
-                for (int i = 0; i < 16; ++i) m = fminf(m, fmaf(beta, acc[qb][i], cnv[i]));
-                if (__any(m < thr[qb])) {
+                for (int g = 0; g < 4; ++g) {
+                    m[g] = INFINITY;
 #pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        const float sc = fmaf(beta, acc[qb][i], cnv[i]);
-                        const bool pass = sc < thr[qb];
-                        if (__any(pass)) {
-                            if (pass) {
-                                score_insert<KL>(ld[qb], sc);
-                                thr[qb] = ld[qb][KL - 1];
+                    for (int u = 0; u < 4; ++u) m[g] = fminf(m[g], fmaf(beta, acc[qb][4 * g + u], cnv[4 * g + u]));
+                }
+                if (__any(fminf(fminf(m[0], m[1]), fminf(m[2], m[3])) < thr[qb])) {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        if (!__any(m[g] < thr[qb])) continue;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const float sc = fmaf(beta, acc[qb][4 * g + u], cnv[4 * g + u]);
+                            const bool pass = sc < thr[qb];
+                            if (__any(pass)) {
+                                if (pass) {
+                                    score_insert<KL>(ld[qb], sc);
+                                    thr[qb] = ld[qb][KL - 1];
+                                }
                             }
                         }
                     }
@@ -444,30 +454,48 @@ __global__ __launch_bounds__(256, 2) void k_sym_main(const void *__restrict__ Xh
                 // query side: score of candidate r for q is cn[r] + beta <q, r>. Candidate side:
                 // score of q for r is cn[q] + beta <q, r>; filtered with beta <q, r> - T[r] <=
                 // sigma - cn[q], then tested exactly
-                float m1 = INFINITY, m2 = INFINITY;
+                // the minima of the lane's four groups of four candidates (i = 4 g .. 4 g + 3): the
+                // emission walks only the groups in which some lane of the wave passes
+                float m1[4], m2[4];
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    m1 = fminf(m1, fmaf(beta, acc[qb][i], cnv[i]));
-                    if (!diag) m2 = fminf(m2, fmaf(beta, acc[qb][i], -tv[i]));
-                }
-                if (__any(m1 <= tq[qb])) {
+                for (int g = 0; g < 4; ++g) {
+                    m1[g] = INFINITY;
+                    m2[g] = INFINITY;
 #pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        const int off = 8 * (i >> 2) + 4 * h + (i & 3);
-                        const float s1 = fmaf(beta, acc[qb][i], cnv[i]);
-                        if (s1 <= tq[qb] && c0 + off < Ni && qi[qb] < Ni)
-                            sym_queue(sq, buf, cnt, bucket, B, qi[qb], c0 + off, s1);
+                    for (int u = 0; u < 4; ++u) {
+                        const int i = 4 * g + u;
+                        m1[g] = fminf(m1[g], fmaf(beta, acc[qb][i], cnv[i]));
+                        if (!diag) m2[g] = fminf(m2[g], fmaf(beta, acc[qb][i], -tv[i]));
                     }
                 }
-                if (!diag && __any(m2 <= tcs[qb])) {
+                if (__any(fminf(fminf(m1[0], m1[1]), fminf(m1[2], m1[3])) <= tq[qb])) {
 #pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        const int off = 8 * (i >> 2) + 4 * h + (i & 3);
-                        if (fmaf(beta, acc[qb][i], -tv[i]) <= tcs[qb]) {
-                            const float s2 = fmaf(beta, acc[qb][i], cnq[qb]);
-                            // padding queries (cn = +inf) would pass against an unbounded row
-                            if (c0 + off < Ni && qi[qb] < Ni && s2 <= tv[i])
-                                sym_queue(sq, buf, cnt, bucket, B, c0 + off, qi[qb], s2);
+                    for (int g = 0; g < 4; ++g) {
+                        if (!__any(m1[g] <= tq[qb])) continue;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int i = 4 * g + u;
+                            const int off = 8 * g + 4 * h + u;
+                            const float s1 = fmaf(beta, acc[qb][i], cnv[i]);
+                            if (s1 <= tq[qb] && c0 + off < Ni && qi[qb] < Ni)
+                                sym_queue(sq, buf, cnt, bucket, B, qi[qb], c0 + off, s1);
+                        }
+                    }
+                }
+                if (!diag && __any(fminf(fminf(m2[0], m2[1]), fminf(m2[2], m2[3])) <= tcs[qb])) {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        if (!__any(m2[g] <= tcs[qb])) continue;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int i = 4 * g + u;
+                            const int off = 8 * g + 4 * h + u;
+                            if (fmaf(beta, acc[qb][i], -tv[i]) <= tcs[qb]) {
+                                const float s2 = fmaf(beta, acc[qb][i], cnq[qb]);
+                                // padding queries (cn = +inf) would pass against an unbounded row
+                                if (c0 + off < Ni && qi[qb] < Ni && s2 <= tv[i])
+                                    sym_queue(sq, buf, cnt, bucket, B, c0 + off, qi[qb], s2);
+                            }
                         }
                     }
                 }

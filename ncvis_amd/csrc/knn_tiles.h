@@ -196,19 +196,29 @@ __global__ __launch_bounds__(256) void k_knn_tiles(const void *__restrict__ Xh_,
             }
 #pragma unroll
             for (int qb = 0; qb < QB; ++qb) {
-                float m = INFINITY;
+                // minima of the lane's four groups of four scores: the insertion walks only the
+                // groups in which some lane beats its (falling) threshold, in the same order
+                float m[4];
 #pragma unroll
-                for (int i = 0; i < 16; ++i) m = fminf(m, fmaf(beta, acc[qb][i], cnv[i]));
-                if (__any(m < thr[qb])) {
+                for (int g = 0; g < 4; ++g) {
+                    m[g] = INFINITY;
 #pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        float sc = fmaf(beta, acc[qb][i], cnv[i]);
-                        bool pass = sc < thr[qb];
-                        if (__any(pass)) {
-                            if (pass) {
-                                int ci = (int)(c0 + 8 * (i >> 2) + 4 * h + (i & 3));
-                                list_insert<KL>(ld[qb], li[qb], sc, ci);
-                                thr[qb] = ld[qb][KL - 1];
+                    for (int u = 0; u < 4; ++u) m[g] = fminf(m[g], fmaf(beta, acc[qb][4 * g + u], cnv[4 * g + u]));
+                }
+                if (__any(fminf(fminf(m[0], m[1]), fminf(m[2], m[3])) < thr[qb])) {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        if (!__any(m[g] < thr[qb])) continue;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            float sc = fmaf(beta, acc[qb][4 * g + u], cnv[4 * g + u]);
+                            bool pass = sc < thr[qb];
+                            if (__any(pass)) {
+                                if (pass) {
+                                    int ci = (int)(c0 + 8 * g + 4 * h + u);
+                                    list_insert<KL>(ld[qb], li[qb], sc, ci);
+                                    thr[qb] = ld[qb][KL - 1];
+                                }
                             }
                         }
                     }
@@ -373,17 +383,24 @@ __global__ __launch_bounds__(256) void k_knn_tiles_dma(const void *__restrict__ 
                 cnv[4 * g + 3] = v.w;                                                                  \
             }                                                                                          \
             _Pragma("unroll") for (int qb = 0; qb < QB; ++qb) {                                        \
-                float m = INFINITY;                                                                    \
-                _Pragma("unroll") for (int i = 0; i < 16; ++i) m = fminf(m, fmaf(beta, acc[qb][i], cnv[i])); \
-                if (__any(m < thr[qb])) {                                                              \
-                    _Pragma("unroll") for (int i = 0; i < 16; ++i) {                                   \
-                        float sc = fmaf(beta, acc[qb][i], cnv[i]);                                     \
-                        bool pass = sc < thr[qb];                                                      \
-                        if (__any(pass)) {                                                             \
-                            if (pass) {                                                                \
-                                int ci = (int)(c0 + 8 * (i >> 2) + 4 * h + (i & 3));                   \
-                                list_insert<KL>(ld[qb], li[qb], sc, ci);                               \
-                                thr[qb] = ld[qb][KL - 1];                                              \
+                float m[4];                                                                            \
+                _Pragma("unroll") for (int g = 0; g < 4; ++g) {                                        \
+                    m[g] = INFINITY;                                                                   \
+                    _Pragma("unroll") for (int u = 0; u < 4; ++u) m[g] =                               \
+                        fminf(m[g], fmaf(beta, acc[qb][4 * g + u], cnv[4 * g + u]));                   \
+                }                                                                                      \
+                if (__any(fminf(fminf(m[0], m[1]), fminf(m[2], m[3])) < thr[qb])) {                    \
+                    _Pragma("unroll") for (int g = 0; g < 4; ++g) {                                    \
+                        if (!__any(m[g] < thr[qb])) continue;                                          \
+                        _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                \
+                            float sc = fmaf(beta, acc[qb][4 * g + u], cnv[4 * g + u]);                 \
+                            bool pass = sc < thr[qb];                                                  \
+                            if (__any(pass)) {                                                         \
+                                if (pass) {                                                            \
+                                    int ci = (int)(c0 + 8 * g + 4 * h + u);                            \
+                                    list_insert<KL>(ld[qb], li[qb], sc, ci);                           \
+                                    thr[qb] = ld[qb][KL - 1];                                          \
+                                }                                                                      \
                             }                                                                          \
                         }                                                                              \
                     }                                                                                  \
