@@ -43,7 +43,7 @@ typedef struct ncvis_handle ncvis_handle;
 /* ncvis::NCVis::NCVis (src/ncvis.hpp:54). n_noise may be NULL (3 per epoch, src/ncvis.cpp:16-20)
  * and is copied; entries must be below 2^32 - 1. n_threads is the reference's OpenMP thread
  * count. Each GPU SGD chain plays one such thread (its own pcg64(random_seed + chain), a static
- * chunk of the edges, a private Q copy). By default the chain count is GPU-sized (E / 64, at most
+ * chunk of the edges, a private Q copy). By default the chain count is GPU-sized (E / 32, at most
  * 2^17) whatever n_threads says, because a few sequential chains leave the GPU idle. Option
  * "exact_threads" = 1 runs exactly n_threads chains instead; n_threads = 1 then reproduces the
  * reference at one thread. M and ef_construction are accepted for signature compatibility; the

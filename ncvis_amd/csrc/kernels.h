@@ -22,7 +22,7 @@ struct Tuning {
                                      // (0: at row changes and sub-epoch ends)
     int sgd_fast_math = 1;           // many-chain epochs: single-precision powf/expf (0: double, rounded
                                      // once, as the exact one-chain mode always does)
-    int64_t edges_per_chain = 64;    // auto chain count = E / edges_per_chain,
+    int64_t edges_per_chain = 32;    // auto chain count = E / edges_per_chain,
     int64_t min_chains = 1;          // ... at least min(E / 16, min_chains) (1: off) ...
     int64_t max_chains = 1 << 17;    // ... and at most this
     int knn_dma = -1;                // FP32 kNN tiles: stages by LDS DMA (1 or auto -1) or registers (0)
