@@ -68,8 +68,10 @@ const char *ncvis_last_error(const ncvis_handle *h);
  *                     and the copies are merged after every one of "sgd_sync" sub-epochs
  *   "exact_threads"   1: run exactly n_threads chains (default 0: GPU-sized chain count)
  *   "chains"          override the SGD chain count (0 = none)
- *   "graph"           1: ncvis_fit_transform_many captures its work in a hipGraph and replays it
- *                     (default), 0: queues it eagerly on every call */
+ *   "graph"           1 (default): ncvis_fit_transform_many captures its work in a hipGraph and
+ *                     replays it; a device-pointer fit repeated on the same buffers and shape is
+ *                     captured at its second call and replayed from then on (stage times included);
+ *                     0: every fit is queued eagerly */
 int ncvis_set_option(ncvis_handle *h, const char *key, double value);
 
 /* Process-wide tuning knobs (no reference counterpart). None changes a single-chain fit; the kNN
@@ -106,7 +108,8 @@ int ncvis_fit_transform_many(ncvis_handle *h, size_t count, const float *const *
                              float *const *dY, int n_streams, void *stream);
 
 /* Stage times of the last fit (milliseconds): [knn, symmetrize, init, optimize, total, knn_tiles,
- * knn_rerank, sgd_kernel] (sgd_kernel: the epoch kernel launches alone, summed), the edge
+ * knn_rerank, sgd_kernel] (sgd_kernel: the epoch kernel launches alone, summed; NaN for a fit
+ * replayed from its captured graph, which records the stages but not every launch), the edge
  * count E, the final Q and the effective chain count. */
 int ncvis_get_stats(const ncvis_handle *h, double *stage_ms8, int64_t *n_edges, float *final_Q, int64_t *chains);
 

@@ -2,6 +2,7 @@
 // and the device-pointer stage entry points. Declarations and reference citations are in
 // include/ncvis_amd.h.
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 #include <atomic>
 #include <memory>
@@ -28,7 +29,8 @@ struct ncvis_handle {
                               // stores, 1 device atomics, 2 per-XCD copies of Y
     int64_t chains_override = 0;
     int exact_threads = 0;  // 1: exactly n_threads chains (chain t = the reference's thread t)
-    int use_graph = 1;      // fit_transform_many: capture the batch in a hipGraph and replay it
+    int use_graph = 1;      // replay fits from a captured hipGraph (fit_transform_many: every call;
+                            // single fits: from the second call on the same shape and buffers)
     uint64_t gen = 1;       // bumped by every option change
     std::unique_ptr<struct FitBatch> single, batch;  // plans kept for the next fit of the same shapes
     std::string last_error;
@@ -125,6 +127,8 @@ struct FitEvents {
     hipEvent_t stage[5];  // start, after knn, after symmetrize, after init, end
     KnnEvents knn;
     std::vector<hipEvent_t> epochs;
+    bool per_launch = true;  // record the epoch launches too (not in a graph: ~2,000 event nodes cost
+                             // more than the replay saves)
     FitEvents(int n_launches) : epochs(2 * (size_t)std::max(n_launches, 0), nullptr) {
         for (auto &e : stage) e = nullptr;
         knn = KnnEvents{nullptr, nullptr, nullptr};
@@ -198,21 +202,26 @@ struct FitBatch {
     int *errs = nullptr;  // [B] kNN flags per descriptor position, then [groups] symmetrize flags
     FitSeg *d_segs = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::unique_ptr<FitEvents> ev;  // single fits: the stage and epoch events (recorded by the graph too)
+    int eager_runs = 0;             // single fits queued eagerly with the current plan and pointers
+    bool no_graph = false;          // a capture failed: single fits stay eager on this plan
 
     FitBatch() = default;
     FitBatch(const FitBatch &) = delete;
     FitBatch &operator=(const FitBatch &) = delete;
-    ~FitBatch() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-    }
+    ~FitBatch() { drop_graph(); }
 
     bool matches(const std::vector<FitJob> &js, uint64_t g) const { return g == gen && js == jobs; }
 
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        eager_runs = 0;
+    }
+
     void plan(const ncvis_handle *h, const std::vector<FitJob> &js, uint64_t g) {
-        if (exec) {
-            (void)hipGraphExecDestroy(exec);
-            exec = nullptr;
-        }
+        drop_graph();
+        no_graph = false;
         arena.release();
         jobs = js;
         gen = g;
@@ -324,6 +333,7 @@ struct FitBatch {
 
     // new input and output pointers for the same shapes (single fits reuse their arena)
     void repoint(const std::vector<FitJob> &js) {
+        drop_graph();  // the graph holds the old pointers
         jobs = js;
         for (size_t p = 0; p < segs.size(); ++p) segs[p].Y = jobs[order[p]].dY;
     }
@@ -339,7 +349,7 @@ struct FitBatch {
         const int d = (int)h->d;
         k_zero_ints<<<cdiv(B + G, 256), 256, 0, st>>>(errs, B + G);
         NCVIS_LAUNCH_CHECK();
-        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[0], st));
+        if (ev) record_event(ev->stage[0], st);
         for (int gi = 0; gi < G; ++gi) {
             const Group &gr = groups[gi];
             for (int p = gr.b0; p < gr.b0 + gr.nb; ++p) {
@@ -349,7 +359,7 @@ struct FitBatch {
                             dists + local * gr.k, knn_ws, errs + p, st, ev ? &ev->knn : nullptr, nullptr,
                             (int32_t)local);
             }
-            if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[1], st));
+            if (ev) record_event(ev->stage[1], st);
             if (gr.nb == 1) {
                 symmetrize_enqueue(inds, dists, gr.rows, gr.k, const_cast<int64_t *>(segs[gr.b0].row_ptr),
                                    const_cast<int32_t *>(segs[gr.b0].col), cdist, sym_ws, errs + B + gi, st);
@@ -359,10 +369,10 @@ struct FitBatch {
                 NCVIS_LAUNCH_CHECK();
             }
         }
-        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[2], st));
+        if (ev) record_event(ev->stage[2], st);
         seg_dims(d_segs, B, rule_of(h), d, st);
         seg_init_embedding(d_segs, B, tot, d, h->n_init_epochs, h->random_seed, st);
-        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[3], st));
+        if (ev) record_event(ev->stage[3], st);
         SgdParams p;
         p.a = h->a;
         p.b = h->b;
@@ -373,18 +383,23 @@ struct FitBatch {
         p.n_noise = h->n_noise.data();
         p.chains = 0;  // per data set, on the device
         p.atomic_updates = h->atomic_updates;
-        seg_optimize(d_segs, B, tot, d, 0.f, p, st, ev ? ev->epochs.data() : nullptr);  // Q starts at 0 (ncvis.cpp:417)
-        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->stage[4], st));
+        seg_optimize(d_segs, B, tot, d, 0.f, p, st, ev && ev->per_launch ? ev->epochs.data() : nullptr);  // Q starts at 0 (ncvis.cpp:417)
+        if (ev) record_event(ev->stage[4], st);
     }
 
-    void capture(const ncvis_handle *h, hipStream_t cap) {
+    // the whole queue of enqueue() as one graph, captured on the handle's own stream (the
+    // caller's stream may carry other work); event records become graph nodes
+    void capture(const ncvis_handle *h, hipStream_t cap, FitEvents *fev = nullptr) {
         upload(cap);
         NCVIS_HIP_CHECK(hipStreamSynchronize(cap));
         hipGraph_t g = nullptr;
         NCVIS_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
         try {
-            enqueue(h, cap, nullptr);
+            if (fev) fev->per_launch = false;
+            enqueue(h, cap, fev);
+            if (fev) fev->per_launch = true;
         } catch (...) {
+            if (fev) fev->per_launch = true;
             (void)hipStreamEndCapture(cap, &g);
             if (g) (void)hipGraphDestroy(g);
             throw;
@@ -426,6 +441,17 @@ struct FitStats {
     int mode = 0;
     int launches = 0;  // epoch kernel launches (epochs x sub-epochs)
 };
+
+// Elapsed time between two recorded events; NaN, with the runtime's error cleared, when it has no
+// timestamps for them (stage times are statistics, never a reason to fail a fit).
+static double event_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return std::nan("");
+    }
+    return ms;
+}
 
 // One whole fit (ncvis.cpp:354-459) on device buffers, queued on `st`, timed stage by stage.
 static void fit_device(ncvis_handle *h, const float *dX, size_t N, size_t D, float *dY, hipStream_t st,
@@ -520,30 +546,53 @@ static void fit_device(ncvis_handle *h, const float *dX, size_t N, size_t D, flo
     FitBatch &fb = *h->single;
     const bool same_shape = fb.gen == g && fb.jobs.size() == 1 && fb.jobs[0].N == N && fb.jobs[0].D == D &&
                             (((uintptr_t)fb.jobs[0].dY % 8 == 0) == ((uintptr_t)dY % 8 == 0));
-    if (same_shape) {
-        fb.repoint(js);  // same arena, new input and output pointers
-    } else {
+    if (!same_shape) {
         fb.plan(h, js, g);
+    } else if (!(fb.jobs[0].dX == dX && fb.jobs[0].dY == dY)) {
+        fb.repoint(js);  // same arena, new input and output pointers
     }
     const int launches = h->n_epochs * sgd_sub_epochs(fb.tot);
-    FitEvents ev(launches);
-    fb.upload(st);
-    fb.enqueue(h, st, &ev);
+    if (!fb.ev || fb.ev->epochs.size() != 2 * (size_t)launches) {
+        fb.drop_graph();  // a captured graph records the old events
+        fb.ev.reset(new FitEvents(launches));
+    }
+    FitEvents &ev = *fb.ev;
+    // A fit repeated on the same buffers replays one captured graph: eagerly, its ~2,000 launches
+    // (1,000 at 1M: epochs and merges) are issued one host call at a time, and small fits wait on
+    // the host. The first fit on a plan and pointers runs eagerly, so one-off fits never pay for a
+    // capture.
+    if (h->use_graph && fb.eager_runs > 0 && !fb.exec && !fb.no_graph) {
+        try {
+            fb.capture(h, h->stream, &ev);
+        } catch (const std::exception &) {
+            // a runtime that cannot capture this queue keeps the fit eager, as with option "graph" 0
+            fb.drop_graph();
+            fb.no_graph = true;
+            fb.eager_runs = 1;
+            (void)hipGetLastError();
+        }
+    }
+    const bool replay = h->use_graph && fb.exec;
+    if (replay) {
+        NCVIS_HIP_CHECK(hipGraphLaunch(fb.exec, st));
+    } else {
+        fb.upload(st);
+        fb.enqueue(h, st, &ev);
+        fb.eager_runs += 1;
+    }
     fb.check_errors(st, false);  // synchronises st
     FitSeg v;
     NCVIS_HIP_CHECK(hipMemcpy(&v, fb.d_segs, sizeof(FitSeg), hipMemcpyDeviceToHost));
     float Q = 0.f;
     NCVIS_HIP_CHECK(hipMemcpy(&Q, fb.segs[0].qdev, sizeof(float), hipMemcpyDeviceToHost));
-    float ms[4];
-    for (int i = 0; i < 4; ++i) NCVIS_HIP_CHECK(hipEventElapsedTime(&ms[i], ev.stage[i], ev.stage[i + 1]));
-    for (int i = 0; i < 4; ++i) out.ms[i] = ms[i];
-    out.ms[4] = ms[0] + ms[1] + ms[2] + ms[3];
-    float t1 = 0.f, t2 = 0.f;
-    NCVIS_HIP_CHECK(hipEventElapsedTime(&t1, ev.knn.tiles_begin, ev.knn.tiles_end));
-    NCVIS_HIP_CHECK(hipEventElapsedTime(&t2, ev.knn.tiles_end, ev.knn.rerank_end));
-    out.ms[5] = t1;
-    out.ms[6] = t2;
-    out.ms[7] = epoch_time_ms(ev.epochs.data(), launches);
+    for (int i = 0; i < 4; ++i) out.ms[i] = event_ms(ev.stage[i], ev.stage[i + 1]);
+    out.ms[4] = out.ms[0] + out.ms[1] + out.ms[2] + out.ms[3];
+    out.ms[5] = event_ms(ev.knn.tiles_begin, ev.knn.tiles_end);
+    out.ms[6] = event_ms(ev.knn.tiles_end, ev.knn.rerank_end);
+    double sgd = replay ? std::nan("") : 0.0;  // a replay records the stages, not every epoch launch
+    if (!replay)
+        for (int l = 0; l < launches; ++l) sgd += event_ms(ev.epochs[2 * (size_t)l], ev.epochs[2 * (size_t)l + 1]);
+    out.ms[7] = sgd;
     out.launches = launches;
     out.n_edges = v.E;
     out.Q = Q;

@@ -24,6 +24,26 @@ struct HipError : std::runtime_error {
 
 #define NCVIS_LAUNCH_CHECK() NCVIS_HIP_CHECK(hipGetLastError())
 
+// Records e on st. On a stream under capture the record becomes an explicit event-record node of
+// the graph being captured (after everything captured so far, before what follows), so every
+// replay records e again and the stage times of a replayed fit are its own.
+inline void record_event(hipEvent_t e, hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    NCVIS_HIP_CHECK(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusActive) {
+        NCVIS_HIP_CHECK(hipEventRecord(e, st));
+        return;
+    }
+    unsigned long long id = 0;
+    hipGraph_t g = nullptr;
+    const hipGraphNode_t *deps = nullptr;
+    size_t ndeps = 0;
+    NCVIS_HIP_CHECK(hipStreamGetCaptureInfo_v2(st, &cs, &id, &g, &deps, &ndeps));
+    hipGraphNode_t node = nullptr;
+    NCVIS_HIP_CHECK(hipGraphAddEventRecordNode(&node, g, deps, ndeps, e));
+    NCVIS_HIP_CHECK(hipStreamUpdateCaptureDependencies(st, &node, 1, hipStreamSetCaptureDependencies));
+}
+
 // Stream for stream-ordered allocations made by this host thread (null: hipMalloc / hipFree).
 // Batched fits run one data set per worker thread and stream; hipFree would wait for every
 // stream of the device and serialise them.

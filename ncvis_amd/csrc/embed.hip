@@ -808,7 +808,7 @@ void seg_optimize(const FitSeg *d_segs, int B, const SegTotals &tot, int d, floa
         c.edge_flush = tu.sgd_edge_flush;
         for (int sub = 0; sub < S; ++sub) {
             hipEvent_t *ev = epoch_events ? epoch_events + 2 * ((size_t)epoch * S + sub) : nullptr;
-            if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev[0], st));
+            if (ev) record_event(ev[0], st);
             switch (d) {
                 case 1: launch_epoch_seg<1>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, sub, S, st); break;
                 case 2: launch_epoch_seg<2>(tu.sgd_prefetch, grid, d_segs, B, tot.slots, c, d, tu.sgd_plain_far, sub, S, st); break;
@@ -819,7 +819,7 @@ void seg_optimize(const FitSeg *d_segs, int B, const SegTotals &tot, int d, floa
                     break;
             }
             NCVIS_LAUNCH_CHECK();
-            if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev[1], st));
+            if (ev) record_event(ev[1], st);
             if (rep) rep_sync(d_segs, B, tot.rows, d, 0, st);
         }
         k_q_reduce_seg<<<B, 1024, 0, st>>>(d_segs, epoch);

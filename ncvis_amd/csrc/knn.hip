@@ -680,7 +680,7 @@ void knn_enqueue(const float *dX, int64_t N, int D, int k, int dist, int prec, i
         }
         NCVIS_LAUNCH_CHECK();
         const float beta = is_l2 ? -2.0f : -1.0f;
-        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->tiles_begin, st));
+        if (ev) record_event(ev->tiles_begin, st);
         if (L.wide32)
             knn_tiles_wide_f32(KL, DK, N, Npad, xh, cn, beta, cand, st, S);
         else if (L.wide)
@@ -701,7 +701,7 @@ void knn_enqueue(const float *dX, int64_t N, int D, int k, int dist, int prec, i
         else
             knn_tiles_q1_fp16(KL, DK, N, Npad, xh, cn, beta, cand, st, S);
         NCVIS_LAUNCH_CHECK();
-        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->tiles_end, st));
+        if (ev) record_event(ev->tiles_end, st);
         const int vec4 = (D % 4 == 0) && ((uintptr_t)Xp % 16 == 0);
         if (L.C == 32)
             k_knn_rerank<32><<<cdiv(N, 8), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand, k, d_inds, d_dists, d_err,
@@ -713,15 +713,15 @@ void knn_enqueue(const float *dX, int64_t N, int D, int k, int dist, int prec, i
             k_knn_rerank128<<<cdiv(N, 4), 256, 0, st>>>(Xp, N, D, is_l2, vec4, cand, k, d_inds, d_dists, d_err,
                                                             label_off);
         NCVIS_LAUNCH_CHECK();
-        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->rerank_end, st));
+        if (ev) record_event(ev->rerank_end, st);
     } else {
-        if (ev) NCVIS_HIP_CHECK(hipEventRecord(ev->tiles_begin, st));
+        if (ev) record_event(ev->tiles_begin, st);
         k_knn_generic<<<cdiv(N, 256), 256, 0, st>>>(Xp, N, D, is_l2, k, (float *)(w + L.hd), (int *)(w + L.hi),
                                                     d_inds, d_dists, label_off);
         NCVIS_LAUNCH_CHECK();
         if (ev) {
-            NCVIS_HIP_CHECK(hipEventRecord(ev->tiles_end, st));
-            NCVIS_HIP_CHECK(hipEventRecord(ev->rerank_end, st));
+            record_event(ev->tiles_end, st);
+            record_event(ev->rerank_end, st);
         }
     }
 }

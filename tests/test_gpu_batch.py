@@ -186,3 +186,48 @@ def test_graph_replays_match_eager(torch_cuda):
             assert np.array_equal(e.view(np.uint32), y.cpu().numpy().view(np.uint32)), rep
         h.fit_transform_device(other.data_ptr(), 700, 16, yo.data_ptr())
         torch.cuda.synchronize()
+
+
+def test_single_fit_graph_replays_match_eager(torch_cuda):
+    # a device-pointer fit repeated on the same buffers runs eagerly once, is captured at its
+    # second call and replayed from then on: every replay gives the eager result bit for bit (one
+    # chain: deterministic), with eager fits of another shape in between, and records its own
+    # stage times (the epoch launches' sum is NaN for a replay)
+    torch = torch_cuda
+    from ncvis_amd import abi
+    print("[case] single-fit graph replays", flush=True)
+    X = _sets([(1800, 48, 6)], seed=21)[0]
+    dX = torch.from_numpy(X).cuda()
+    dY = torch.empty((1800, 2), dtype=torch.float32, device="cuda")
+    h = abi.Handle(d=2, n_threads=1, n_neighbors=15, n_epochs=3, n_init_epochs=4, a=0.77, b=1.22)
+    h.set_option("exact_threads", 1)
+    h.fit_transform_device(dX.data_ptr(), 1800, 48, dY.data_ptr())
+    torch.cuda.synchronize()
+    eager = dY.cpu().numpy().copy()
+    s0 = h.stats()["stage_ms"]
+    assert all(np.isfinite(v) and v >= 0 for v in s0.values())
+    other = torch.from_numpy(_sets([(700, 16, 3)], seed=9)[0]).cuda()
+    yo = torch.empty((700, 2), dtype=torch.float32, device="cuda")
+    h2 = abi.Handle(d=2, n_threads=1, n_neighbors=15, n_epochs=3, n_init_epochs=4, a=0.77, b=1.22)
+    totals = []
+    for rep in range(4):
+        dY.fill_(float("nan"))
+        h.fit_transform_device(dX.data_ptr(), 1800, 48, dY.data_ptr())
+        torch.cuda.synchronize()
+        assert np.array_equal(eager.view(np.uint32), dY.cpu().numpy().view(np.uint32)), rep
+        s = h.stats()["stage_ms"]
+        for name, v in s.items():
+            if name == "sgd_kernel":
+                assert np.isnan(v), (rep, v)
+            else:
+                assert np.isfinite(v) and v >= 0, (rep, name, v)
+        totals.append(s["total"])
+        h2.fit_transform_device(other.data_ptr(), 700, 16, yo.data_ptr())
+        torch.cuda.synchronize()
+    assert len(set(totals)) > 1, totals  # recorded by each replay, not left from the eager fit
+    # option "graph" 0: eager again, every launch timed
+    h.set_option("graph", 0)
+    h.fit_transform_device(dX.data_ptr(), 1800, 48, dY.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(eager.view(np.uint32), dY.cpu().numpy().view(np.uint32))
+    assert np.isfinite(h.stats()["stage_ms"]["sgd_kernel"])
