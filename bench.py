@@ -477,7 +477,8 @@ def main():
                "knn_recall@15": round(h_recall, 5)}
 
     out = {
-        "metric": "fit_transform sec @1M×128 (mixed vs FP32 GPU vs CPU); recall@15 vs FP64",
+        "metric": ("fit_transform sec @1M×128 (mixed vs FP32 GPU vs CPU); recall@15 vs FP64" if args.workload == "1m"
+                   else f"fit_transform sec @{N // 1000}k×{D}" if N >= 10_000 else f"fit_transform sec @{N}×{D}"),
         "value": round(sec, 4),
         "unit": "s",
         "n_gpus": 1,
