@@ -371,13 +371,22 @@ def main():
         thr = (s_last["chains"] // 256) * 256
         cal = abi.calibrate_rmw(N, thr, 512)
         copies = abi.calibrate_rmw_copies(N, thr, 512)
+        eighths = abi.calibrate_rmw_eighths(N, thr, 512)
         samples_per_s = n_samples / sgd_s
-        ceiling = copies if s_last["sgd_mode"] == 3 else cal["load_cas_rows_per_s"]
+        # copies larger than an XCD's 4 MB L2 hold the rows at permuted positions and a launch's
+        # negatives on an XCD come from one eighth of them (tuning knob sgd_neg_slices, default 1)
+        permuted = (s_last["sgd_mode"] == 3 and N * d * 4 > (4 << 20) and tuned.get("sgd_neg_slices", 1) != 0)
+        if s_last["sgd_mode"] != 3:
+            ceiling, pattern = cal["load_cas_rows_per_s"], "device-scope 64-bit CAS"
+        elif permuted:
+            ceiling, pattern = eighths, "per-XCD copy re-read + store, rows from the XCD's eighth"
+        else:
+            ceiling, pattern = copies, "per-XCD copy re-read + store"
         roof_sgd["rmw_ceiling"] = {"samples_per_s": samples_per_s,
-                                   "pattern": "per-XCD copy re-read + store" if s_last["sgd_mode"] == 3
-                                   else "device-scope 64-bit CAS",
+                                   "pattern": pattern,
                                    "ceiling_rows_per_s": ceiling,
                                    "copy_rows_per_s": copies,
+                                   "copy_eighths_rows_per_s": eighths,
                                    "load_cas_rows_per_s": cal["load_cas_rows_per_s"],
                                    "load_rows_per_s": cal["load_rows_per_s"],
                                    "frac": samples_per_s / ceiling}

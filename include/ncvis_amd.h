@@ -180,9 +180,12 @@ const char *ncvis_autotune_last_error(void);
  * read-only and once with a device-scope 64-bit compare-and-swap update; outputs rows per
  * second. ncvis_calibrate_rmw_copies measures the per-XCD-copy epoch's pattern: the table is
  * held once per XCD, each wave reads a random row of its XCD's copy, reads it again and stores
- * the update (workgroup scope, served by that XCD's L2). */
+ * the update (workgroup scope, served by that XCD's L2). ncvis_calibrate_rmw_eighths is the same
+ * with each XCD's rows drawn from its own eighth of the copy, the negatives' pattern of the epochs
+ * on permuted copies (option sgd_neg_slices; rows >= 8). */
 int ncvis_calibrate_rmw(size_t rows, int64_t threads, int iters, double *load_rows_per_s, double *cas_rows_per_s);
 int ncvis_calibrate_rmw_copies(size_t rows, int64_t threads, int iters, double *rows_per_s);
+int ncvis_calibrate_rmw_eighths(size_t rows, int64_t threads, int iters, double *rows_per_s);
 const char *ncvis_calibrate_last_error(void);
 
 /* ncvis_optimize_device without any device allocation. No counterpart in the reference: it is

@@ -19,7 +19,7 @@ EXPORTS = (
     "ncvis_nce_loss_device", "ncvis_tune", "ncvis_fit_transform_many", "ncvis_optimize_device_noalloc",
     "ncvis_autotune_knn", "ncvis_load_tune_cache", "ncvis_tune_cache_size", "ncvis_autotune_last_error",
     "ncvis_knn_plan",
-    "ncvis_calibrate_rmw", "ncvis_calibrate_rmw_copies", "ncvis_calibrate_last_error",
+    "ncvis_calibrate_rmw", "ncvis_calibrate_rmw_copies", "ncvis_calibrate_rmw_eighths", "ncvis_calibrate_last_error",
 )
 
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
@@ -74,6 +74,7 @@ def lib():
         L.ncvis_calibrate_rmw.argtypes = [C.c_size_t, C.c_int64, C.c_int, C.POINTER(C.c_double),
                                           C.POINTER(C.c_double)]
         L.ncvis_calibrate_rmw_copies.argtypes = [C.c_size_t, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+        L.ncvis_calibrate_rmw_eighths.argtypes = [C.c_size_t, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         L.ncvis_calibrate_last_error.argtypes = []
         L.ncvis_calibrate_last_error.restype = C.c_char_p
         L.ncvis_tune_cache_size.argtypes = []
@@ -136,6 +137,15 @@ def calibrate_rmw_copies(rows: int, threads: int, iters: int = 512) -> float:
     XCD's own copy of a table of `rows` 8-byte entries."""
     r = C.c_double()
     if lib().ncvis_calibrate_rmw_copies(rows, threads, iters, C.byref(r)) != 0:
+        raise RuntimeError(lib().ncvis_calibrate_last_error().decode())
+    return r.value
+
+
+def calibrate_rmw_eighths(rows: int, threads: int, iters: int = 512) -> float:
+    """The per-XCD-copy update with each XCD's rows drawn from its own eighth of the copy (the
+    negatives of the epochs on permuted copies), rows/s."""
+    r = C.c_double()
+    if lib().ncvis_calibrate_rmw_eighths(rows, threads, iters, C.byref(r)) != 0:
         raise RuntimeError(lib().ncvis_calibrate_last_error().decode())
     return r.value
 

@@ -182,6 +182,40 @@ __global__ void k_split_csr(const FitSeg *__restrict__ s, int B, int64_t base_ro
 // of all data sets run segmented, one launch each. The memory is one arena that stays allocated
 // for the next fit of the same shapes; a batch can be captured in a hipGraph and replayed while
 // its inputs stay the same.
+// Multiplier of the affine bijection r -> a r mod N of permuted copies: the integer nearest
+// N (sqrt(5) - 1) / 2 that is coprime to N (consecutive rows land about 0.618 N apart, so any run
+// of rows spreads evenly over the positions).
+static uint64_t perm_multiplier(uint64_t N) {
+    uint64_t a = (uint64_t)((double)N * 0.6180339887498949);
+    if (a < 1) a = 1;
+    auto gcd = [](uint64_t x, uint64_t y) {
+        while (y) {
+            const uint64_t t = x % y;
+            x = y;
+            y = t;
+        }
+        return x;
+    };
+    while (gcd(a, N) != 1) ++a;
+    return a % N;
+}
+
+// a^-1 mod N (a coprime to N), extended Euclid
+static uint64_t mod_inverse(uint64_t a, uint64_t N) {
+    int64_t t = 0, nt = 1, r = (int64_t)N, nr = (int64_t)(a % N);
+    while (nr != 0) {
+        const int64_t q = r / nr;
+        int64_t x = t - q * nt;
+        t = nt;
+        nt = x;
+        x = r - q * nr;
+        r = nr;
+        nr = x;
+    }
+    if (t < 0) t += (int64_t)N;
+    return (uint64_t)t;
+}
+
 struct FitBatch {
     std::vector<FitJob> jobs;   // in the caller's order
     std::vector<int> order;     // descriptor position -> caller's index (sorted by k, stable)
@@ -299,7 +333,12 @@ struct FitBatch {
             off[p].rng = take(sizeof(U128) * v.T_cap);
             off[p].qpart = take(sizeof(float) * v.T_cap);
             off[p].qdev = take(sizeof(float) * (1 + (size_t)std::max(h->n_epochs, 0)));
-            off[p].yrep = take(reps ? sizeof(float) * kXcds * N * d : 0);
+            // copies larger than an XCD's L2: rows at permuted positions, negatives by eighths
+            const bool perm = reps && tuning().sgd_neg_slices && (size_t)N * d * sizeof(float) > (4u << 20);
+            v.perm_a = perm ? perm_multiplier((uint64_t)N) : 0;
+            v.perm_ai = perm ? mod_inverse(v.perm_a, (uint64_t)N) : 0;
+            tot.permuted = tot.permuted || perm;
+            off[p].yrep = take(reps ? sizeof(float) * (kXcds + (perm ? 1 : 0)) * N * d : 0);
             off[p].sub_row = take(sizeof(int64_t) * v.T_cap);
         }
         const size_t o_errs = take(sizeof(int) * (B + groups.size())), o_segs = take(sizeof(FitSeg) * B);
@@ -694,6 +733,8 @@ int ncvis_tune(const char *key, double value) {
         } else if (k == "sgd_replicas") {
             if (value != 0 && value != 1) throw std::invalid_argument("[ncvis_amd] sgd_replicas: 0 or 1");
             t.sgd_replicas = (int)value;
+        } else if (k == "sgd_neg_slices") {
+            t.sgd_neg_slices = value != 0;
         } else if (k == "sgd_edge_flush") {
             t.sgd_edge_flush = value != 0;
         } else if (k == "sgd_sync") {

@@ -24,7 +24,8 @@ namespace ncvis_hip {
 // MODE 0: load only; 1: load + device-scope 64-bit CAS on one table (the device-atomic epoch);
 // 2: the per-XCD-copy epoch's update: the table holds kXcds copies, a wave works on its XCD's
 // copy (hardware XCC id), reads the row, reads it again and stores the update, all at workgroup
-// scope (served by the XCD's L2).
+// scope (served by the XCD's L2); 3: the same with the rows drawn from the eighth of the copy
+// given by the XCD, the negatives' pattern of the epochs on permuted copies.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_rmw_probe(unsigned long long *table, uint64_t rows, int iters,
                                                    float *sink) {
@@ -32,14 +33,19 @@ __global__ __launch_bounds__(256) void k_rmw_probe(unsigned long long *table, ui
     U128 g = pcg_seed(777 + (uint64_t)t);
     float acc = 0.f;
     unsigned long long *tab = table;
-    if (MODE == 2) {
+    uint64_t lo = 0, range = rows;
+    if (MODE >= 2) {
         const int x = __builtin_amdgcn_s_getreg((20 /* HW_REG_XCC_ID */) | (0 << 6) | (15 << 11)) & (kXcds - 1);
         tab = table + (uint64_t)x * rows;
+        if (MODE == 3) {
+            lo = rows * (uint64_t)x / kXcds;
+            range = rows * (uint64_t)(x + 1) / kXcds - lo;
+        }
     }
     for (int it = 0; it < iters; ++it) {
-        const uint64_t r = uniform_index(g, rows);
+        const uint64_t r = lo + uniform_index(g, range);
         unsigned long long *addr = tab + r;
-        if (MODE == 2) {
+        if (MODE >= 2) {
             const unsigned long long first = __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             acc += __builtin_bit_cast(float2, first).x;
             const unsigned long long cur = __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -85,7 +91,8 @@ static double probe_rate(int mode, size_t rows, int64_t threads, int iters) {
         hipStream_t s;
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } sg{st};
-    const size_t entries = rows * (mode == 2 ? kXcds : 1);
+    if (mode == 3 && rows < (size_t)kXcds) throw std::invalid_argument("[ncvis_amd] calibrate: rows < 8");
+    const size_t entries = rows * (mode >= 2 ? kXcds : 1);
     DevBuf<unsigned long long> table(entries);
     DevBuf<float> sink(1);
     k_rmw_fill<<<cdiv((int64_t)entries, 256), 256, 0, st>>>(table.p, entries);
@@ -108,8 +115,10 @@ static double probe_rate(int mode, size_t rows, int64_t threads, int iters) {
             k_rmw_probe<0><<<grid, 256, 0, st>>>(table.p, rows, iters, sink.p);
         else if (mode == 1)
             k_rmw_probe<1><<<grid, 256, 0, st>>>(table.p, rows, iters, sink.p);
-        else
+        else if (mode == 2)
             k_rmw_probe<2><<<grid, 256, 0, st>>>(table.p, rows, iters, sink.p);
+        else
+            k_rmw_probe<3><<<grid, 256, 0, st>>>(table.p, rows, iters, sink.p);
         NCVIS_LAUNCH_CHECK();
         NCVIS_HIP_CHECK(hipEventRecord(e1, st));
         NCVIS_HIP_CHECK(hipEventSynchronize(e1));
@@ -151,6 +160,13 @@ int ncvis_calibrate_rmw(size_t rows, int64_t threads, int iters, double *load_ro
 int ncvis_calibrate_rmw_copies(size_t rows, int64_t threads, int iters, double *rows_per_s) {
     return cal_guarded([&] {
         const double r = probe_rate(2, rows, threads, iters);
+        if (rows_per_s) *rows_per_s = r;
+    });
+}
+
+int ncvis_calibrate_rmw_eighths(size_t rows, int64_t threads, int iters, double *rows_per_s) {
+    return cal_guarded([&] {
+        const double r = probe_rate(3, rows, threads, iters);
         if (rows_per_s) *rows_per_s = r;
     });
 }

@@ -307,6 +307,7 @@ struct SampleConst {
     uint32_t noise;    // cur_noise
     int fast_math;     // 1: single-precision powf / expf
     int edge_flush;    // mode 3: write the source row's accumulated update after every edge
+    uint32_t rot;      // mode 3 with permuted copies: launch count that rotates the negatives' eighths
 };
 
 // The sample body of ncvis.cpp:304-339 on a register copy of the two rows. Returns the weight
@@ -493,6 +494,11 @@ __device__ __forceinline__ void rep_store(float *p, const float (&v)[DIM]) {
     }
 }
 
+// Hardware id of the XCD (L2 domain) this wave runs on.
+__device__ __forceinline__ int xcd_id() {
+    return __builtin_amdgcn_s_getreg((20 /* HW_REG_XCC_ID */) | (0 << 6) | (15 << 11)) & (kXcds - 1);
+}
+
 // One chain of the many-chain (Hogwild) modes. The chain keeps the row of its current source
 // point in registers and adds its own updates of that row there (it sees its own writes, as a
 // thread of the reference does), flushing them to Y when it moves to the next row; updates of the
@@ -507,11 +513,25 @@ __device__ __forceinline__ void rep_store(float *p, const float (&v)[DIM]) {
 //     written, so only a chain of the same XCD updating that row in between loses its update
 //     (the reference's Hogwild race, narrowed to one L2 round trip). The copies are merged after
 //     every sub-epoch (k_rep_sync).
+//   pa != 0 (mode 3 with negative slices): the copies hold row r at position (pa r) mod N, an
+//     affine bijection that scatters any run of rows (a cluster of rows sorted by class, say)
+//     evenly over the positions; a launch draws its negatives on an XCD as positions of one
+//     eighth of the copy, which stays in that XCD's L2, every eighth in turn (c.rot), so each
+//     negative is a uniform row over the launches and independent of the input order.
 template <int DIM, int PF, bool REP>
 __device__ __forceinline__ void chain_hw(float *__restrict__ Y, const int64_t *__restrict__ row_ptr,
                                          const int32_t *__restrict__ col, int64_t s0, int64_t e0, int64_t &r,
-                                         U128 &g, float &Qc, const SampleConst &c, uint64_t N, int plain_far) {
+                                         U128 &g, float &Qc, const SampleConst &c, uint64_t N, int plain_far,
+                                         uint64_t pa = 0) {
+    // position of a row in Y (the identity unless the copies are permuted)
+    auto pos = [&](int64_t row) -> int64_t {
+        if constexpr (REP) {
+            if (pa) return (int64_t)(((uint64_t)row * pa) % N);
+        }
+        return row;
+    };
     int64_t rend = row_ptr[r + 1];
+    int64_t rp = pos(r);  // position of the source row
     float base[DIM], acc[DIM];
     auto flush = [&](int64_t row) {
         if constexpr (REP) {
@@ -540,20 +560,29 @@ __device__ __forceinline__ void chain_hw(float *__restrict__ Y, const int64_t *_
             for (int k = 0; k < DIM; ++k) v[k] = Y[row * DIM + k];
         }
     };
-    fetch(r, base);
+    fetch(rp, base);
 #pragma unroll
     for (int k = 0; k < DIM; ++k) acc[k] = 0.f;
     const uint32_t total = c.noise + 1;
+    // negatives are positions in [nlo, nlo + nrange): all of Y, or with permuted copies the
+    // eighth given by this XCD and launch
+    uint64_t nlo = 0, nrange = N;
+    if (REP && pa) {
+        const uint64_t sl = (uint64_t)((xcd_id() + c.rot) & (kXcds - 1));
+        nlo = N * sl / kXcds;
+        nrange = N * (sl + 1) / kXcds - nlo;
+    }
     for (int64_t e = s0; e < e0; ++e) {
         while (e >= rend) {
-            flush(r);
+            flush(rp);
             ++r;
             rend = row_ptr[r + 1];
-            fetch(r, base);
+            rp = pos(r);
+            fetch(rp, base);
 #pragma unroll
             for (int k = 0; k < DIM; ++k) acc[k] = 0.f;
         }
-        const int64_t id = r;
+        const int64_t id = rp;  // positions from here on
         for (uint32_t j0 = 0; j0 < total; j0 += PF) {
             int64_t oth[PF];
             float yo[PF][DIM];
@@ -563,11 +592,11 @@ __device__ __forceinline__ void chain_hw(float *__restrict__ Y, const int64_t *_
                 oth[p] = id;
                 if (j < total) {
                     if (j == 0) {
-                        oth[p] = col[e];
+                        oth[p] = pos(col[e]);
                     } else {
                         int64_t o;
                         do {
-                            o = (int64_t)uniform_index(g, N);
+                            o = (int64_t)(nlo + uniform_index(g, nrange));
                         } while (o == id);
                         oth[p] = o;
                     }
@@ -629,25 +658,21 @@ __device__ __forceinline__ void chain_hw(float *__restrict__ Y, const int64_t *_
                 for (int k = 0; k < DIM; ++k) any |= acc[k] != 0.f;
                 if (any) {
                     float cur[DIM];
-                    rep_load<DIM>(Y + r * DIM, cur);
+                    rep_load<DIM>(Y + rp * DIM, cur);
 #pragma unroll
                     for (int k = 0; k < DIM; ++k) {
                         cur[k] = __fadd_rn(cur[k], acc[k]);
                         base[k] = cur[k];
                         acc[k] = 0.f;
                     }
-                    rep_store<DIM>(Y + r * DIM, cur);
+                    rep_store<DIM>(Y + rp * DIM, cur);
                 }
             }
         }
     }
-    flush(r);
+    flush(rp);
 }
 
-// Hardware id of the XCD (L2 domain) this wave runs on.
-__device__ __forceinline__ int xcd_id() {
-    return __builtin_amdgcn_s_getreg((20 /* HW_REG_XCC_ID */) | (0 << 6) | (15 << 11)) & (kXcds - 1);
-}
 
 // Sub-epoch `sub` of `S` of every chain of every data set: thread = chain slot (slots past a
 // data set's T exit); chain t takes part sub of its static edge chunk, so S consecutive launches
@@ -676,7 +701,7 @@ __global__ __launch_bounds__(256) void k_sgd_epoch_seg(const FitSeg *__restrict_
     if constexpr (DIM >= 1 && DIM <= 4) {
         if (mode == 3)
             chain_hw<DIM, PF, true>(g.Yrep + (int64_t)xcd_id() * g.N * DIM, g.row_ptr, g.col, s0, e0, r, rng, Qc, c,
-                                    N, 0);
+                                    N, 0, g.perm_a);
         else if (mode == 2)
             chain_hw<DIM, PF, false>(g.Y, g.row_ptr, g.col, s0, e0, r, rng, Qc, c, N, plain_far);
         else
@@ -692,6 +717,9 @@ __global__ __launch_bounds__(256) void k_sgd_epoch_seg(const FitSeg *__restrict_
 // Per-XCD copies of Y (mode 3 data sets): init = 1 copies Y into every copy; otherwise the copies
 // are merged, Y + sum over XCDs of (copy - Y) in a fixed order (every update of every copy kept
 // once), and the result is written to Y and to every copy. One thread per row.
+// With permuted copies (perm_a != 0) thread i handles position i: the base of the merges is the
+// permuted Y kept after the copies (Yrep + kXcds N d); init = 1 gathers it from Y (row
+// perm_ai i mod N), and k_rep_final scatters it back after the last epoch.
 template <int DIM>
 __global__ __launch_bounds__(256) void k_rep_sync(const FitSeg *__restrict__ s, int B, int64_t rows, int init) {
     const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -700,9 +728,18 @@ __global__ __launch_bounds__(256) void k_rep_sync(const FitSeg *__restrict__ s, 
     if (g.mode != 3) return;
     const int64_t i = gi - g.row_off;
     const int64_t stride = g.N * DIM;
+    float *base = g.perm_a ? g.Yrep + kXcds * stride : g.Y;
     float y[DIM];
+    if (g.perm_a && init) {
+        const int64_t row = (int64_t)(((uint64_t)i * g.perm_ai) % (uint64_t)g.N);
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) y[k] = g.Y[i * DIM + k];
+        for (int k = 0; k < DIM; ++k) y[k] = g.Y[row * DIM + k];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) base[i * DIM + k] = y[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) y[k] = base[i * DIM + k];
+    }
     if (!init) {
         float v[DIM];
 #pragma unroll
@@ -715,7 +752,7 @@ __global__ __launch_bounds__(256) void k_rep_sync(const FitSeg *__restrict__ s, 
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
             y[k] = v[k];
-            g.Y[i * DIM + k] = v[k];
+            base[i * DIM + k] = v[k];
         }
     }
 #pragma unroll
@@ -723,6 +760,20 @@ __global__ __launch_bounds__(256) void k_rep_sync(const FitSeg *__restrict__ s, 
 #pragma unroll
         for (int k = 0; k < DIM; ++k) g.Yrep[x * stride + i * DIM + k] = y[k];
     }
+}
+
+// Permuted copies: Y[row] = the merged base at the row's position, after the last merge.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_rep_final(const FitSeg *__restrict__ s, int B, int64_t rows) {
+    const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= rows) return;
+    const FitSeg &g = s[seg_find<0>(s, B, gi)];
+    if (g.mode != 3 || !g.perm_a) return;
+    const int64_t i = gi - g.row_off;
+    const float *base = g.Yrep + kXcds * g.N * DIM;
+    const int64_t row = (int64_t)(((uint64_t)i * g.perm_ai) % (uint64_t)g.N);
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) g.Y[row * DIM + k] = base[i * DIM + k];
 }
 
 // One 1024-thread block per data set; each thread keeps four loads in flight so 2^17 chain
@@ -782,6 +833,18 @@ static void rep_sync(const FitSeg *segs, int B, int64_t rows, int d, int init, h
     NCVIS_LAUNCH_CHECK();
 }
 
+static void rep_final(const FitSeg *segs, int B, int64_t rows, int d, hipStream_t st) {
+    const int nb = (int)cdiv(rows, 256);
+    switch (d) {
+        case 1: k_rep_final<1><<<nb, 256, 0, st>>>(segs, B, rows); break;
+        case 2: k_rep_final<2><<<nb, 256, 0, st>>>(segs, B, rows); break;
+        case 3: k_rep_final<3><<<nb, 256, 0, st>>>(segs, B, rows); break;
+        case 4: k_rep_final<4><<<nb, 256, 0, st>>>(segs, B, rows); break;
+        default: return;
+    }
+    NCVIS_LAUNCH_CHECK();
+}
+
 int sgd_sub_epochs(const SegTotals &tot) { return tot.sub_rows ? std::max(1, tuning().sgd_sync) : 1; }
 
 void seg_optimize(const FitSeg *d_segs, int B, const SegTotals &tot, int d, float Q0, const SgdParams &p,
@@ -807,6 +870,7 @@ void seg_optimize(const FitSeg *d_segs, int B, const SegTotals &tot, int d, floa
         c.fast_math = tu.sgd_fast_math;
         c.edge_flush = tu.sgd_edge_flush;
         for (int sub = 0; sub < S; ++sub) {
+            c.rot = (uint32_t)(3 * epoch + sub);
             hipEvent_t *ev = epoch_events ? epoch_events + 2 * ((size_t)epoch * S + sub) : nullptr;
             if (ev) record_event(ev[0], st);
             switch (d) {
@@ -825,6 +889,7 @@ void seg_optimize(const FitSeg *d_segs, int B, const SegTotals &tot, int d, floa
         k_q_reduce_seg<<<B, 1024, 0, st>>>(d_segs, epoch);
         NCVIS_LAUNCH_CHECK();
     }
+    if (rep && tot.permuted) rep_final(d_segs, B, tot.rows, d, st);
 }
 
 // One data set with host-known E and chain count (the stage entry points).

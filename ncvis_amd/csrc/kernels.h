@@ -18,6 +18,7 @@ struct Tuning {
                                      // atomics are automatic, 0 never (device atomics, mode 2)
     int sgd_sync = 20;               // sub-epochs per epoch: every chain writes its source row out and,
                                      // with copies, the copies are merged after each
+    int sgd_neg_slices = 1;          // per-XCD copies beyond an XCD L2: rows at permuted positions, negatives by eighths
     int sgd_edge_flush = 0;          // per-XCD copies: write the source row's update after every edge
                                      // (0: at row changes and sub-epoch ends)
     int sgd_fast_math = 1;           // many-chain epochs: single-precision powf/expf (0: double, rounded
@@ -184,6 +185,10 @@ struct FitSeg {
     float *Yrep;               // [kXcds, N, d] per-XCD copies of Y (mode 3), or null
     int64_t *sub_row;          // [T_cap] row of each chain's last edge between sub-epochs (null: one
                                // launch per epoch)
+    uint64_t perm_a, perm_ai;  // mode 3, copies larger than an XCD's L2 (sgd_neg_slices): the copies
+                               // hold row r at position perm_a r mod N (perm_ai inverts it) and
+                               // Yrep holds kXcds + 1 arrays (the last: the merges' base); 0: rows
+                               // in place
 };
 
 // XCDs (L2 domains) of the part; the per-XCD copies of Y are indexed by the hardware XCC id.
@@ -195,6 +200,7 @@ struct SegTotals {
     int max_nb = 1;         // largest nb
     bool replicas = false;  // some data set has per-XCD copies of Y (merged after every sub-epoch)
     bool sub_rows = false;  // every data set has sub_row: epochs may run as several sub-epochs
+    bool permuted = false;  // some data set has permuted copies (perm_a != 0)
 };
 
 // Writes one descriptor from the host value (a kernel argument: no copy, capturable).

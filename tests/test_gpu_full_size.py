@@ -140,3 +140,76 @@ def test_fit_full_size(torch_cuda, data):
     agree = (lab[ey] == lab[q][:, None]).float().mean().item()
     print(f"1M fit: {s['stage_ms']}, Q {s['Q']:.4f}, label agreement@15 {agree:.4f}", flush=True)
     assert agree >= 0.99, agree
+
+
+def _fit(torch, abi, dX, alpha=1.0, slices=1, seed=42):
+    from ncvis_amd._ncvis import find_ab_params, negative_plan_for
+    a, b = find_ab_params(1.0, 0.4)
+    plan, _ = negative_plan_for(None, 50)
+    try:
+        abi.tune("sgd_neg_slices", slices)
+        h = abi.Handle(d=2, n_threads=0, n_neighbors=K, random_seed=seed, n_init_epochs=20, a=a, b=b, alpha=alpha,
+                       n_noise=list(plan))
+        dY = torch.empty((N, 2), dtype=torch.float32, device="cuda")
+        h.fit_transform_device(dX.data_ptr(), N, D, dY.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+    finally:
+        abi.tune("sgd_neg_slices", 1)
+    return dY, h.stats(), a, b, plan
+
+
+def test_permuted_copies_round_trip(torch_cuda, data):
+    # with the step at 0 every update is 0: the embedding after the epochs is the init layout,
+    # whether the copies hold the rows in place or at permuted positions (gathered before the
+    # first epoch, merged, scattered back after the last), bit for bit
+    torch = torch_cuda
+    from ncvis_amd import abi
+    dX, _ = data
+    _say("1M fit at alpha=0: permuted copies vs rows in place")
+    y0, s0, _, _, _ = _fit(torch, abi, dX, alpha=0.0, slices=0)
+    y1, s1, _, _, _ = _fit(torch, abi, dX, alpha=0.0, slices=1)
+    assert s0["sgd_mode"] == 3 and s1["sgd_mode"] == 3
+    assert torch.equal(y0, y1)
+
+
+def test_negative_eighths_on_sorted_rows(torch_cuda, data):
+    # rows sorted by mixture component (each component a contiguous run): the negatives drawn from
+    # one eighth of the permuted positions per XCD and launch give the loss (at each run's own
+    # final Q, over one fixed table of negatives) of negatives drawn from all rows, within 2 %,
+    # and the same cluster structure
+    torch = torch_cuda
+    from ncvis_amd import abi
+    dX0, labels0 = data
+    o = np.argsort(labels0, kind="stable")
+    dX = dX0[torch.from_numpy(o).cuda()].contiguous()
+    labels = labels0[o]
+    _say("1M sorted rows: negatives from eighths vs from all rows")
+    inds = torch.empty((N, K), dtype=torch.int32, device="cuda")
+    dists = torch.empty((N, K), dtype=torch.float32, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    abi.knn(dX.data_ptr(), N, D, K, "euclidean", "fp16", inds.data_ptr(), dists.data_ptr(), st)
+    rp = torch.empty(N + 1, dtype=torch.int64, device="cuda")
+    col = torch.empty(2 * N * K, dtype=torch.int32, device="cuda")
+    cd = torch.empty(2 * N * K, dtype=torch.float32, device="cuda")
+    E = abi.symmetrize(inds.data_ptr(), dists.data_ptr(), N, K, rp.data_ptr(), col.data_ptr(), cd.data_ptr(), st)
+    del cd
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    out = {}
+    for sl in (0, 1):
+        y, s, a, b, plan = _fit(torch, abi, dX, slices=sl)
+        n_neg = int(plan[-1])
+        if sl == 0:
+            neg = torch.randint(0, N, (E, n_neg), dtype=torch.int32, device="cuda", generator=gen)
+        L = torch.zeros(1, dtype=torch.float64, device="cuda")
+        abi.nce_loss(N, 2, rp.data_ptr(), col.data_ptr(), E, neg.data_ptr(), n_neg, y.data_ptr(), s["Q"], a, b,
+                     L.data_ptr(), None, None, st)
+        torch.cuda.synchronize()
+        q = torch.from_numpy(np.random.default_rng(8).choice(N, 2000, replace=False)).cuda()
+        ey = _exact(torch, y.double(), q, K)
+        lab = torch.from_numpy(labels.astype(np.int64)).cuda()
+        agree = (lab[ey] == lab[q][:, None]).float().mean().item()
+        out[sl] = (float(L.item()), s["Q"], agree, s["stage_ms"]["optimize"])
+    print(f"sorted rows: all rows {out[0]}, eighths {out[1]}", flush=True)
+    assert abs(out[1][0] - out[0][0]) <= 0.02 * out[0][0]
+    assert out[0][2] >= 0.99 and out[1][2] >= 0.99
