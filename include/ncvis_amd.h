@@ -76,7 +76,8 @@ int ncvis_set_option(ncvis_handle *h, const char *key, double value);
 
 /* Process-wide tuning knobs (no reference counterpart). None changes a single-chain fit; the kNN
  * knobs never change the kNN table. Keys: "sgd_prefetch" (1,2,4,8), "sgd_plain_far" (0/1),
- * "sgd_fast_math" (0/1), "sgd_replicas" (1: many-chain epochs on per-XCD copies of Y when the
+ * "sgd_fast_math" (many-chain sample math: 2 hardware log2/exp2/reciprocal (default), 1 powf/expf,
+ * 0 double), "sgd_replicas" (1: many-chain epochs on per-XCD copies of Y when the
  * atomics are automatic, 0: device atomics), "sgd_sync" (sub-epochs per epoch, 1..64: every chain
  * writes its source row out after each, and the copies are merged), "sgd_edge_flush" (0/1: with
  * copies, the source row is written after every edge), "edges_per_chain" (auto chain count = E / this, at least

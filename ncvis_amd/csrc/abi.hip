@@ -729,7 +729,8 @@ int ncvis_tune(const char *key, double value) {
         } else if (k == "sgd_plain_far") {
             t.sgd_plain_far = value != 0;
         } else if (k == "sgd_fast_math") {
-            t.sgd_fast_math = value != 0;
+            if (value != 0 && value != 1 && value != 2) throw std::invalid_argument("[ncvis_amd] sgd_fast_math: 0, 1 or 2");
+            t.sgd_fast_math = (int)value;
         } else if (k == "sgd_replicas") {
             if (value != 0 && value != 1) throw std::invalid_argument("[ncvis_amd] sgd_replicas: 0 or 1");
             t.sgd_replicas = (int)value;

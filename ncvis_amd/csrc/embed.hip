@@ -305,7 +305,7 @@ struct SampleConst {
     float a, b, alpha_Q, step;
     float noise_f;     // (float)cur_noise
     uint32_t noise;    // cur_noise
-    int fast_math;     // 1: single-precision powf / expf
+    int fast_math;     // 1: single-precision powf / expf; 2: hardware log2 / exp2 / reciprocal
     int edge_flush;    // mode 3: write the source row's accumulated update after every edge
     uint32_t rot;      // mode 3 with permuted copies: launch count that rotates the negatives' eighths
 };
@@ -320,6 +320,34 @@ __device__ __forceinline__ void sgd_math(float (&yi)[DIM], float (&yo)[DIM], boo
     for (int k = 0; k < DIM; ++k) {
         float t = __fsub_rn(yi[k], yo[k]);
         d2 = __fadd_rn(d2, __fmul_rn(t, t));
+    }
+    if (c.fast_math == 2) {
+        // hardware log2 / exp2 / reciprocal (about 1 ulp each): d2^b and d2^(b-1) share one log2,
+        // e^Q = 2^(Q log2 e). d2 = 0 gives log2 = -inf, so d2^b = 0 and d2^(b-1) = 0 or +inf as
+        // powf gives; b = 1 keeps d2^0 = 1
+        const float lg = __builtin_amdgcn_logf(d2);
+        const float Ph = __builtin_amdgcn_rcpf(__fadd_rn(1.f, __fmul_rn(c.a, __builtin_amdgcn_exp2f(c.b * lg))));
+        float w = 1.f;
+        if (c.noise != 0) {
+            w = Ph * __builtin_amdgcn_rcpf(c.noise_f * __builtin_amdgcn_exp2f(Qc * 1.44269504088896341f));
+            if (positive)
+                w = __builtin_amdgcn_rcpf(1.f + w);
+            else
+                w = -__builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_rcpf(w));
+            Qc = __fsub_rn(Qc, __fmul_rn(w, c.alpha_Q));
+            const float pb1 = c.b == 1.f ? 1.f : __builtin_amdgcn_exp2f((c.b - 1.f) * lg);
+            w = 2.f * w * Ph * c.a * c.b * pb1;
+        }
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            float v = __fmul_rn(__fmul_rn(__fsub_rn(yo[k], yi[k]), w), c.step);
+            if (v > 4.f)
+                v = 4.f;
+            else if (v < -4.f)
+                v = -4.f;
+            dx[k] = v;
+        }
+        return;
     }
     float Ph = 1.f / __fadd_rn(1.f, __fmul_rn(c.a, c.fast_math ? powf(d2, c.b) : pow_rn(d2, c.b)));
     float w = 1.f;

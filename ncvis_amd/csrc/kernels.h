@@ -21,8 +21,9 @@ struct Tuning {
     int sgd_neg_slices = 1;          // per-XCD copies beyond an XCD L2: rows at permuted positions, negatives by eighths
     int sgd_edge_flush = 0;          // per-XCD copies: write the source row's update after every edge
                                      // (0: at row changes and sub-epoch ends)
-    int sgd_fast_math = 1;           // many-chain epochs: single-precision powf/expf (0: double, rounded
-                                     // once, as the exact one-chain mode always does)
+    int sgd_fast_math = 2;           // many-chain epochs: 2 hardware log2/exp2/reciprocal, 1 single-
+                                     // precision powf/expf, 0 double rounded once (as the exact one-chain
+                                     // mode always does)
     int64_t edges_per_chain = 32;    // auto chain count = E / edges_per_chain,
     int64_t min_chains = 1;          // ... at least min(E / 16, min_chains) (1: off) ...
     int64_t max_chains = 1 << 17;    // ... and at most this
