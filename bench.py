@@ -379,7 +379,7 @@ def main():
         if s_last["sgd_mode"] != 3:
             ceiling, pattern = cal["load_cas_rows_per_s"], "device-scope 64-bit CAS"
         elif permuted:
-            ceiling, pattern = eighths, "per-XCD copy re-read + store, rows from the XCD's eighth"
+            ceiling, pattern = eighths, "per-XCD copy load + store, rows from the XCD's eighth"
         else:
             ceiling, pattern = copies, "per-XCD copy re-read + store"
         roof_sgd["rmw_ceiling"] = {"samples_per_s": samples_per_s,

@@ -80,7 +80,11 @@ int ncvis_set_option(ncvis_handle *h, const char *key, double value);
  * 0 double), "sgd_replicas" (1: many-chain epochs on per-XCD copies of Y when the
  * atomics are automatic, 0: device atomics), "sgd_sync" (sub-epochs per epoch, 1..64: every chain
  * writes its source row out after each, and the copies are merged), "sgd_edge_flush" (0/1: with
- * copies, the source row is written after every edge), "edges_per_chain" (auto chain count = E / this, at least
+ * copies, the source row is written after every edge), "sgd_reread" (per-XCD copies: 1 read the
+ * far row again right before its store, 0 store the prefetched row minus the update, -1 (default)
+ * 0 on permuted copies, 1 otherwise), "sgd_neg_slices" (1 (default): copies larger than an XCD's
+ * L2 hold the rows at permuted positions and a launch draws an XCD's negatives from one eighth of
+ * them; 0: rows in place, negatives over all rows), "edges_per_chain" (auto chain count = E / this, at least
  * min(E / 16, "min_chains"), at most "max_chains"),
  * "knn_dma" (FP32 tiles: -1 auto = 1 LDS DMA, 0 register-staged), "knn_stage_mult" (16-bit
  * tiles: 1 or 2 x the LDS stage rows), "knn_split" (candidate ranges per query block: 0 auto,
@@ -181,9 +185,9 @@ const char *ncvis_autotune_last_error(void);
  * read-only and once with a device-scope 64-bit compare-and-swap update; outputs rows per
  * second. ncvis_calibrate_rmw_copies measures the per-XCD-copy epoch's pattern: the table is
  * held once per XCD, each wave reads a random row of its XCD's copy, reads it again and stores
- * the update (workgroup scope, served by that XCD's L2). ncvis_calibrate_rmw_eighths is the same
- * with each XCD's rows drawn from its own eighth of the copy, the negatives' pattern of the epochs
- * on permuted copies (option sgd_neg_slices; rows >= 8). */
+ * the update (workgroup scope, served by that XCD's L2). ncvis_calibrate_rmw_eighths measures the
+ * epochs' pattern on permuted copies (option sgd_neg_slices; rows >= 8): each XCD's rows drawn
+ * from its own eighth of the copy, each read once and stored. */
 int ncvis_calibrate_rmw(size_t rows, int64_t threads, int iters, double *load_rows_per_s, double *cas_rows_per_s);
 int ncvis_calibrate_rmw_copies(size_t rows, int64_t threads, int iters, double *rows_per_s);
 int ncvis_calibrate_rmw_eighths(size_t rows, int64_t threads, int iters, double *rows_per_s);

@@ -142,8 +142,8 @@ def calibrate_rmw_copies(rows: int, threads: int, iters: int = 512) -> float:
 
 
 def calibrate_rmw_eighths(rows: int, threads: int, iters: int = 512) -> float:
-    """The per-XCD-copy update with each XCD's rows drawn from its own eighth of the copy (the
-    negatives of the epochs on permuted copies), rows/s."""
+    """The epochs' update on permuted copies: each XCD's rows drawn from its own eighth of the
+    copy, each read once and stored, rows/s."""
     r = C.c_double()
     if lib().ncvis_calibrate_rmw_eighths(rows, threads, iters, C.byref(r)) != 0:
         raise RuntimeError(lib().ncvis_calibrate_last_error().decode())

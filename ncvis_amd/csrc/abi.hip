@@ -736,6 +736,9 @@ int ncvis_tune(const char *key, double value) {
             t.sgd_replicas = (int)value;
         } else if (k == "sgd_neg_slices") {
             t.sgd_neg_slices = value != 0;
+        } else if (k == "sgd_reread") {
+            if (value != -1 && value != 0 && value != 1) throw std::invalid_argument("[ncvis_amd] sgd_reread: -1, 0 or 1");
+            t.sgd_reread = (int)value;
         } else if (k == "sgd_edge_flush") {
             t.sgd_edge_flush = value != 0;
         } else if (k == "sgd_sync") {

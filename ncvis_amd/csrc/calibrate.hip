@@ -24,8 +24,8 @@ namespace ncvis_hip {
 // MODE 0: load only; 1: load + device-scope 64-bit CAS on one table (the device-atomic epoch);
 // 2: the per-XCD-copy epoch's update: the table holds kXcds copies, a wave works on its XCD's
 // copy (hardware XCC id), reads the row, reads it again and stores the update, all at workgroup
-// scope (served by the XCD's L2); 3: the same with the rows drawn from the eighth of the copy
-// given by the XCD, the negatives' pattern of the epochs on permuted copies.
+// scope (served by the XCD's L2); 3: the epochs' pattern on permuted copies: the rows drawn from
+// the eighth of the copy given by the XCD, each read once and stored (no second read).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_rmw_probe(unsigned long long *table, uint64_t rows, int iters,
                                                    float *sink) {
@@ -48,7 +48,8 @@ __global__ __launch_bounds__(256) void k_rmw_probe(unsigned long long *table, ui
         if (MODE >= 2) {
             const unsigned long long first = __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             acc += __builtin_bit_cast(float2, first).x;
-            const unsigned long long cur = __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const unsigned long long cur =
+                MODE == 3 ? first : __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             float2 v = __builtin_bit_cast(float2, cur);
             float2 nv = make_float2(__fsub_rn(v.x, 1e-7f), __fadd_rn(v.y, 1e-7f));
             __hip_atomic_store(addr, __builtin_bit_cast(unsigned long long, nv), __ATOMIC_RELAXED,

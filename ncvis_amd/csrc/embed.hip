@@ -308,6 +308,8 @@ struct SampleConst {
     int fast_math;     // 1: single-precision powf / expf; 2: hardware log2 / exp2 / reciprocal
     int edge_flush;    // mode 3: write the source row's accumulated update after every edge
     uint32_t rot;      // mode 3 with permuted copies: launch count that rotates the negatives' eighths
+    int reread;        // mode 3: 1 read the far row again right before its store, 0 never (store the
+                       // prefetched row minus the update), -1 unless the copies are permuted
 };
 
 // The sample body of ncvis.cpp:304-339 on a register copy of the two rows. Returns the weight
@@ -595,6 +597,11 @@ __device__ __forceinline__ void chain_hw(float *__restrict__ Y, const int64_t *_
     // negatives are positions in [nlo, nlo + nrange): all of Y, or with permuted copies the
     // eighth given by this XCD and launch
     uint64_t nlo = 0, nrange = N;
+    // the far row is read again right before its store (one L2 round trip of race window), except
+    // on permuted copies: there the store writes the row as prefetched minus the update, the
+    // reference's own window (Y[other] read at the sample's start, ncvis.cpp:306-336), which
+    // saves a third of the L2 requests of a sample
+    const bool reread = c.reread > 0 || (c.reread < 0 && pa == 0);
     if (REP && pa) {
         const uint64_t sl = (uint64_t)((xcd_id() + c.rot) & (kXcds - 1));
         nlo = N * sl / kXcds;
@@ -643,7 +650,12 @@ __device__ __forceinline__ void chain_hw(float *__restrict__ Y, const int64_t *_
                     for (int k = 0; k < DIM; ++k) acc[k] = __fadd_rn(acc[k], dx[k]);
                     if constexpr (REP) {
                         float cur[DIM];
-                        rep_load<DIM>(Y + oth[p] * DIM, cur);
+                        if (reread) {
+                            rep_load<DIM>(Y + oth[p] * DIM, cur);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < DIM; ++k) cur[k] = yo[p][k];
+                        }
 #pragma unroll
                         for (int k = 0; k < DIM; ++k) cur[k] = __fsub_rn(cur[k], dx[k]);
                         rep_store<DIM>(Y + oth[p] * DIM, cur);
@@ -897,6 +909,7 @@ void seg_optimize(const FitSeg *d_segs, int B, const SegTotals &tot, int d, floa
         c.noise_f = (float)p.n_noise[epoch];
         c.fast_math = tu.sgd_fast_math;
         c.edge_flush = tu.sgd_edge_flush;
+        c.reread = tu.sgd_reread;
         for (int sub = 0; sub < S; ++sub) {
             c.rot = (uint32_t)(3 * epoch + sub);
             hipEvent_t *ev = epoch_events ? epoch_events + 2 * ((size_t)epoch * S + sub) : nullptr;

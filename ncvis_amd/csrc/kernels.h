@@ -19,6 +19,8 @@ struct Tuning {
     int sgd_sync = 20;               // sub-epochs per epoch: every chain writes its source row out and,
                                      // with copies, the copies are merged after each
     int sgd_neg_slices = 1;          // per-XCD copies beyond an XCD L2: rows at permuted positions, negatives by eighths
+    int sgd_reread = -1;             // per-XCD copies: read the far row again right before its store
+                                     // (1 always, 0 never, -1 unless the copies are permuted)
     int sgd_edge_flush = 0;          // per-XCD copies: write the source row's update after every edge
                                      // (0: at row changes and sub-epoch ends)
     int sgd_fast_math = 2;           // many-chain epochs: 2 hardware log2/exp2/reciprocal, 1 single-

@@ -490,7 +490,8 @@ def test_knn_tuning_knobs_keep_the_table(torch_cuda, knobs, precision):
     assert np.array_equal(gi, oi) and np.array_equal(gd, od)
 
 
-@pytest.mark.parametrize("knobs", [{"sgd_prefetch": 4}, {"sgd_fast_math": 0}, {"sgd_fast_math": 1}, {"max_chains": 64},
+@pytest.mark.parametrize("knobs", [{"sgd_prefetch": 4}, {"sgd_fast_math": 0}, {"sgd_fast_math": 1}, {"sgd_reread": 0},
+                                   {"sgd_reread": 1}, {"max_chains": 64},
                                    {"sgd_edge_flush": 1}, {"sgd_sync": 1}, {"sgd_replicas": 0}])
 def test_sgd_tuning_knobs_run(torch_cuda, knobs):
     # the many-chain epoch variants the SGD knobs select all descend the same objective
@@ -499,7 +500,8 @@ def test_sgd_tuning_knobs_run(torch_cuda, knobs):
     X, inds, dists, row_ptr, col = _digits_graph(N=1500, D=32, k=15, seed=5)
     N = X.shape[0]
     Y0 = oracle.init_embedding(N, 2, row_ptr, col, np.float32(1.0 / 15), 5, 42, 1)
-    defaults = {"sgd_prefetch": 1, "sgd_fast_math": 2, "max_chains": 1 << 17, "sgd_edge_flush": 0, "sgd_sync": 20,
+    defaults = {"sgd_prefetch": 1, "sgd_fast_math": 2, "sgd_reread": -1, "max_chains": 1 << 17, "sgd_edge_flush": 0,
+                "sgd_sync": 20,
                 "sgd_replicas": 1}
     runs = []
     try:
