@@ -16,7 +16,7 @@ struct Tuning {
     int sgd_plain_far = 0;           // 1: far-endpoint updates with plain stores instead of atomics
     int sgd_replicas = 1;            // many-chain epochs on per-XCD copies of Y (mode 3): 1 when the
                                      // atomics are automatic, 0 never (device atomics, mode 2)
-    int sgd_sync = 20;               // sub-epochs per epoch: every chain writes its source row out and,
+    int sgd_sync = 16;               // sub-epochs per epoch: every chain writes its source row out and,
                                      // with copies, the copies are merged after each
     int sgd_neg_slices = 1;          // per-XCD copies beyond an XCD L2: rows at permuted positions, negatives by eighths
     int sgd_reread = -1;             // per-XCD copies: read the far row again right before its store

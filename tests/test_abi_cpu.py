@@ -155,5 +155,5 @@ def test_sgd_copy_knobs_are_validated():
     for key, bad in (("sgd_replicas", 2), ("sgd_replicas", -1), ("sgd_sync", 0), ("sgd_sync", 65)):
         with pytest.raises(RuntimeError):
             abi.tune(key, bad)
-    for key, good in (("sgd_replicas", 1), ("sgd_sync", 20), ("sgd_edge_flush", 0)):
+    for key, good in (("sgd_replicas", 1), ("sgd_sync", 16), ("sgd_edge_flush", 0)):
         abi.tune(key, good)

@@ -380,7 +380,7 @@ def test_sub_epochs_keep_the_single_chain_fit(torch_cuda):
             torch.cuda.synchronize()
             out.append((dY.cpu().numpy(), h.stats()["Q"]))
     finally:
-        abi.tune("sgd_sync", 20)
+        abi.tune("sgd_sync", 16)
     for Y, q in out[1:]:
         assert np.array_equal(Y, out[0][0]) and q == out[0][1]
 
@@ -501,7 +501,7 @@ def test_sgd_tuning_knobs_run(torch_cuda, knobs):
     N = X.shape[0]
     Y0 = oracle.init_embedding(N, 2, row_ptr, col, np.float32(1.0 / 15), 5, 42, 1)
     defaults = {"sgd_prefetch": 1, "sgd_fast_math": 2, "sgd_reread": -1, "max_chains": 1 << 17, "sgd_edge_flush": 0,
-                "sgd_sync": 20,
+                "sgd_sync": 16,
                 "sgd_replicas": 1}
     runs = []
     try:
