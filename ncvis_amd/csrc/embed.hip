@@ -11,7 +11,8 @@
 //                      the thread that owns element i under libgomp's static schedule of the N*d
 //                      loop (ncvis.cpp:198-201); a GPU thread jumps the stream to its element.
 //   k_spmm_seg         Y_new[i] = sum_{edges (i, j) in row order} alpha * Y_old[j], row-owned
-//                      (race-free), uncontracted like the reference's loop (ncvis.cpp:206-218)
+//                      (race-free), uncontracted like the reference's loop (ncvis.cpp:206-218);
+//                      a wave gathers its 64 rows' neighbour rows into LDS, each lane sums its own
 //   k_colstat_seg / k_finalize_seg / k_normalize_seg
 //                      per-dimension mean and population std (FP64 partial sums over a fixed
 //                      block partition, deterministic) and standardisation (ncvis.cpp:220-260)
@@ -112,53 +113,133 @@ __global__ void k_init_random_seg(const FitSeg *__restrict__ s, int B, int64_t t
     }
 }
 
-// One row per thread over the concatenated rows; parity 0 reads Y and writes Y2, 1 the reverse.
-// DIM 1..4: the dimension loop inside the edge loop (each edge's index and row read once, four
-// edges fetched ahead of their in-order accumulation); DIM 0: any d, one dimension at a time.
-// Every dimension's sum keeps the sequential edge order, so both forms give the same bits.
+// One row of the SpMM by one thread, in edge order: each edge's index and row read once, four
+// edges fetched ahead of their in-order accumulation.
 template <int DIM>
-__global__ void k_spmm_seg(const FitSeg *__restrict__ s, int B, int64_t R, int d_rt, int parity) {
-    const int64_t gr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gr >= R) return;
-    const FitSeg &g = s[seg_find<0>(s, B, gr)];
-    const int64_t i = gr - g.row_off;
-    const float *__restrict__ Yo = parity ? g.Y2 : g.Y;
-    float *__restrict__ Yn = parity ? g.Y : g.Y2;
-    const int32_t *__restrict__ col = g.col;
-    const float alpha = g.alpha_init;
-    const int64_t b = g.row_ptr[i], e = g.row_ptr[i + 1];
+__device__ __forceinline__ void spmm_row(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                                         float alpha, const float *__restrict__ Yo, float *__restrict__ Yn,
+                                         int64_t i) {
+    float acc[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) acc[c] = 0.f;
+    const int64_t b = row_ptr[i], e = row_ptr[i + 1];
+    int64_t p = b;
+    for (; p + 4 <= e; p += 4) {
+        int32_t j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = col[p + u];
+        float y[4][DIM];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) y[u][c] = Yo[(int64_t)j[u] * DIM + c];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) acc[c] = __fadd_rn(acc[c], __fmul_rn(alpha, y[u][c]));
+    }
+    for (; p < e; ++p) {
+        const int64_t j = col[p];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) acc[c] = __fadd_rn(acc[c], __fmul_rn(alpha, Yo[j * DIM + c]));
+    }
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) Yn[i * DIM + c] = acc[c];
+}
+
+// Parity 0 reads Y and writes Y2, 1 the reverse. Every dimension's sum keeps the sequential edge
+// order (the reference's), so all forms below give the same bits.
+// DIM 0: any d, one thread per row, one dimension at a time.
+// DIM 1..4: a wave takes 64 consecutive rows of one data set, whose edges form one contiguous
+// range: it reads their column indices coalesced and gathers the neighbour rows 64 at a time into
+// LDS, 256 edges per round, and each lane then adds up the part of its own row in that round. A
+// wave whose rows span two data sets runs one thread per row (spmm_row).
+constexpr int kSpmmChunk = 256;  // edges per wave and round
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_spmm_seg(const FitSeg *__restrict__ s, int B, int64_t R, int d_rt,
+                                                  int parity) {
     if constexpr (DIM == 0) {
+        const int64_t gr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (gr >= R) return;
+        const FitSeg &g = s[seg_find<0>(s, B, gr)];
+        const int64_t i = gr - g.row_off;
+        const float *__restrict__ Yo = parity ? g.Y2 : g.Y;
+        float *__restrict__ Yn = parity ? g.Y : g.Y2;
+        const int64_t b = g.row_ptr[i], e = g.row_ptr[i + 1];
         for (int c = 0; c < d_rt; ++c) {
             float acc = 0.f;
-            for (int64_t p = b; p < e; ++p) acc = __fadd_rn(acc, __fmul_rn(alpha, Yo[(int64_t)col[p] * d_rt + c]));
+            for (int64_t p = b; p < e; ++p)
+                acc = __fadd_rn(acc, __fmul_rn(g.alpha_init, Yo[(int64_t)g.col[p] * d_rt + c]));
             Yn[i * d_rt + c] = acc;
         }
     } else {
+        __shared__ float ly[4][kSpmmChunk * DIM];
+        const int lane = threadIdx.x & 63;
+        const int wave = threadIdx.x >> 6;
+        const int64_t w0 = (int64_t)blockIdx.x * 256 + wave * 64;  // the wave's first row
+        if (w0 >= R) return;                                       // whole wave; no block barrier below
+        const int64_t wl = (w0 + 63 < R) ? w0 + 63 : R - 1;        // its last row
+        const int sb = seg_find<0>(s, B, w0);
+        const int64_t gr = w0 + lane;
+        if (sb != seg_find<0>(s, B, wl)) {  // uniform
+            if (gr < R) {
+                const FitSeg &g = s[seg_find<0>(s, B, gr)];
+                spmm_row<DIM>(g.row_ptr, g.col, g.alpha_init, parity ? g.Y2 : g.Y, parity ? g.Y : g.Y2,
+                              gr - g.row_off);
+            }
+            return;
+        }
+        const FitSeg &g = s[sb];
+        const float *__restrict__ Yo = parity ? g.Y2 : g.Y;
+        float *__restrict__ Yn = parity ? g.Y : g.Y2;
+        const int32_t *__restrict__ col = g.col;
+        const float alpha = g.alpha_init;
+        const int64_t i0 = w0 - g.row_off, nrow = wl - w0 + 1;
+        const int64_t eb = g.row_ptr[i0], ee = g.row_ptr[i0 + nrow];  // the wave's edges
+        const bool live = lane < nrow;
+        const int64_t b = live ? g.row_ptr[i0 + lane] : 0, e = live ? g.row_ptr[i0 + lane + 1] : 0;
+        float *my = ly[wave];
         float acc[DIM];
 #pragma unroll
         for (int c = 0; c < DIM; ++c) acc[c] = 0.f;
-        int64_t p = b;
-        for (; p + 4 <= e; p += 4) {
-            int32_t j[4];
+        constexpr int V = kSpmmChunk / 64;
+        for (int64_t c0 = eb; c0 < ee; c0 += kSpmmChunk) {
+            int32_t j[V];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) j[u] = col[p + u];
-            float y[4][DIM];
+            for (int v = 0; v < V; ++v) {
+                const int64_t p = c0 + v * 64 + lane;
+                j[v] = (p < ee) ? col[p] : -1;
+            }
+            float y[V][DIM];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int v = 0; v < V; ++v)
 #pragma unroll
-                for (int c = 0; c < DIM; ++c) y[u][c] = Yo[(int64_t)j[u] * DIM + c];
+                for (int c = 0; c < DIM; ++c) y[v][c] = (j[v] >= 0) ? Yo[(int64_t)j[v] * DIM + c] : 0.f;
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int v = 0; v < V; ++v)
 #pragma unroll
-                for (int c = 0; c < DIM; ++c) acc[c] = __fadd_rn(acc[c], __fmul_rn(alpha, y[u][c]));
+                for (int c = 0; c < DIM; ++c) my[(v * 64 + lane) * DIM + c] = y[v][c];
+            // the round's rows are in LDS for every lane of the wave
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const int64_t pb = (b > c0) ? b : c0;
+            const int64_t pe = (e < c0 + kSpmmChunk) ? e : c0 + kSpmmChunk;
+            for (int64_t p = pb; p < pe; ++p) {
+#pragma unroll
+                for (int c = 0; c < DIM; ++c)
+                    acc[c] = __fadd_rn(acc[c], __fmul_rn(alpha, my[(int)(p - c0) * DIM + c]));
+            }
+            // every lane's reads of this round precede the next round's writes
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
-        for (; p < e; ++p) {
-            const int64_t j = col[p];
+        if (live) {
 #pragma unroll
-            for (int c = 0; c < DIM; ++c) acc[c] = __fadd_rn(acc[c], __fmul_rn(alpha, Yo[j * DIM + c]));
+            for (int c = 0; c < DIM; ++c) Yn[(i0 + lane) * DIM + c] = acc[c];
         }
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) Yn[i * DIM + c] = acc[c];
     }
 }
 
